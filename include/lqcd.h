@@ -1,0 +1,200 @@
+/*
+ * lqcd.h -- C ABI of the MI355X (gfx950) gauge-update library.
+ *
+ * This is the drop-in boundary under the reference's trait API
+ * (lattice-qcd-rs: LatticeState / LatticeStateWithEField / SymplecticIntegrator
+ * / MonteCarlo).  A Rust FFI crate binds these functions one to one; each
+ * declaration cites the reference interface it replaces.  Plain C types only:
+ * handles, pointers, sizes, doubles.
+ *
+ * Conventions
+ *  - Every function returns an lqcd_status; lqcd_last_error() gives the
+ *    message of the last failure on the calling thread.
+ *  - Host buffers are owned by the caller and use the reference's in-memory
+ *    layouts:
+ *      links  [site][mu] 3x3 complex, COLUMN-major (nalgebra Matrix3), re/im
+ *             interleaved: 18 doubles per link, link index = site*D + mu
+ *             (src/field.rs:584-586, src/lattice.rs:923-927);
+ *      efield [site][mu][8] doubles (src/field.rs:1025-1027);
+ *      site   = sum_i x_i * prod_{k<i} L_k (src/lattice.rs:909-915).
+ *  - Device buffers are owned by the library; fields stay resident in HBM
+ *    between calls.  Calls are synchronous unless stated otherwise.
+ *  - A context owns one HIP stream and is not thread-safe: use one context
+ *    per host thread.
+ */
+#ifndef LQCD_H
+#define LQCD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    LQCD_OK = 0,
+    /* LatticeInitializationError (src/error.rs:219-241) */
+    LQCD_ERR_ZERO_DIMENSION = 1,
+    LQCD_ERR_NON_POSITIVE_SIZE = 2,
+    LQCD_ERR_DIM_TOO_SMALL = 3,
+    /* StateInitializationError::IncompatibleSize (src/error.rs:124-171) */
+    LQCD_ERR_INCOMPATIBLE_SIZE = 4,
+    /* MultiIntegrationError::ZeroIntegration (src/error.rs:93-98) */
+    LQCD_ERR_ZERO_INTEGRATION = 5,
+    LQCD_ERR_INVALID_ARGUMENT = 6,
+    LQCD_ERR_UNSUPPORTED = 7,
+    LQCD_ERR_DEVICE = 8,
+    LQCD_ERR_OUT_OF_MEMORY = 9
+} lqcd_status;
+
+typedef enum { LQCD_F64 = 0, LQCD_F32 = 1 } lqcd_precision;
+
+/* link drift: ADDITIVE = the reference's U + dt * derivative_u
+ * (src/integrator/mod.rs:216-233); EXP = exp(i dt sqrt(6)/a E) U with the
+ * reference's su3_exp_i (src/su3.rs:832-855). */
+typedef enum { LQCD_DRIFT_ADDITIVE = 0, LQCD_DRIFT_EXP = 1 } lqcd_drift;
+
+/* REFERENCE = n calls of integrate_symplectic, kick(dt/2) drift(dt)
+ * kick(dt/2) each (symplectic_euler_rayon.rs:222-252); LEAPFROG merges the
+ * two half kicks between consecutive steps (n + 1 force evaluations). */
+typedef enum { LQCD_SCHEME_REFERENCE = 0, LQCD_SCHEME_LEAPFROG = 1 } lqcd_scheme;
+
+typedef struct lqcd_ctx lqcd_ctx;
+typedef struct lqcd_lattice lqcd_lattice;
+
+/* ---------------------------------------------------------------- misc */
+const char* lqcd_last_error(void);
+const char* lqcd_version(void);
+int lqcd_device_count(int* count);
+
+/* ------------------------------------------------------------- context */
+/* One HIP stream on `device`.  The reference's rayon pool
+ * (src/thread.rs:684-699) is the counterpart. */
+int lqcd_ctx_create(int device, lqcd_ctx** out);
+int lqcd_ctx_destroy(lqcd_ctx* ctx);
+int lqcd_ctx_synchronize(lqcd_ctx* ctx);
+
+/* ------------------------------------------------------------- lattice */
+/* LatticeCyclic::new(size, dim) (src/lattice.rs:190-201), with one extent per
+ * direction (all equal = the reference's hypercubic lattice).  dim in 2..4.
+ * Allocates links (cold start) and a zero E field on the device. */
+int lqcd_lattice_create(lqcd_ctx* ctx, int dim, const size_t* extent, double size,
+                        int precision, lqcd_lattice** out);
+int lqcd_lattice_destroy(lqcd_lattice* lat);
+/* number_of_points / number_of_canonical_links_space (src/lattice.rs:216-245) */
+int lqcd_lattice_info(const lqcd_lattice* lat, size_t* nsite, size_t* nlink);
+
+/* LatticeState::set_link_matrix / link_matrix (src/simulation/state.rs:60-75):
+ * n_links must equal nsite * dim (else LQCD_ERR_INCOMPATIBLE_SIZE). */
+int lqcd_links_upload(lqcd_lattice* lat, const double* links, size_t n_links);
+int lqcd_links_download(lqcd_lattice* lat, double* links, size_t n_links);
+/* LinkMatrix::new_cold (src/field.rs:716-722) */
+int lqcd_links_cold(lqcd_lattice* lat);
+
+/* LatticeStateWithEField::set_e_field / e_field (src/simulation/state.rs:190-200) */
+int lqcd_efield_upload(lqcd_lattice* lat, const double* efield, size_t n_links);
+int lqcd_efield_download(lqcd_lattice* lat, double* efield, size_t n_links);
+/* EField::new_cold (src/field.rs:1131-1141) */
+int lqcd_efield_cold(lqcd_lattice* lat);
+/* Momentum refresh drawn on the device (replaces EField::new_determinist with
+ * Normal(0, sigma), src/field.rs:1086-1099; the reference uses
+ * sigma = 0.5 / beta, src/simulation/state.rs:1097).  Counter-based stream:
+ * Philox4x32-10 keyed by seed, counter selects the draw. */
+int lqcd_efield_random(lqcd_lattice* lat, uint64_t seed, uint64_t counter, double sigma);
+
+/* ---------------------------------------------------------- observables */
+/* LatticeState::average_trace_plaquette (src/simulation/state.rs:115-117,
+ * src/field.rs:775-804): out[0] + i out[1] = sum Tr P / (N D (D-1)/2). */
+int lqcd_average_trace_plaquette(lqcd_lattice* lat, double out[2]);
+/* LatticeState::hamiltonian_links (src/simulation/state.rs:821-849) */
+int lqcd_hamiltonian_links(lqcd_lattice* lat, double beta, double* out);
+/* LatticeStateWithEField::hamiltonian_efield (src/simulation/state.rs:1370-1385) */
+int lqcd_hamiltonian_efield(lqcd_lattice* lat, double beta, double* out);
+
+/* --------------------------------------------------- molecular dynamics */
+/* LatticeStateWithEField::derivative_e for every site
+ * (src/simulation/state.rs:1420-1448); out is [site][mu][8]. */
+int lqcd_derivative_e(lqcd_lattice* lat, double* out, size_t n_links);
+/* integrate_efield over all sites (src/integrator/mod.rs:240-254):
+ * e += dt * derivative_e(U). */
+int lqcd_integrate_efield(lqcd_lattice* lat, double dt);
+/* integrate_link over all links (src/integrator/mod.rs:216-233): U <- drift(U, e, dt). */
+int lqcd_integrate_link(lqcd_lattice* lat, double dt, int drift);
+/* SimulationStateSynchronous::simulate_symplectic_n (src/simulation/state.rs:470-492)
+ * with SymplecticIntegrator::integrate_symplectic
+ * (src/integrator/symplectic_euler_rayon.rs:222-252).  n == 0 ->
+ * LQCD_ERR_ZERO_INTEGRATION. */
+int lqcd_simulate_symplectic_n(lqcd_lattice* lat, double dt, size_t n, int drift, int scheme);
+
+/* ------------------------------------------------------------------ HMC */
+typedef struct {
+    double delta_h;      /* H_new - H_old */
+    double h_old;
+    double h_new;
+    double probability;  /* HybridMonteCarloDiagnostic::prob_replace_last */
+    int accepted;        /* HybridMonteCarloDiagnostic::has_replace_last */
+} lqcd_hmc_stats;
+
+/* One trajectory of HybridMonteCarloDiagnostic::next_element
+ * (src/simulation/monte_carlo/hybrid_monte_carlo.rs:465-471,584-613) with the
+ * momenta already in the lattice's E field (upload or lqcd_efield_random):
+ * H_old, n symplectic steps, H_new, p = clamp(exp(H_old - H_new)) (NaN -> 1),
+ * accept iff p == 1 or uniform < floor(p * 2^64) (rand 0.8 Bernoulli).  On
+ * reject the links are those before the call.  E holds the final momenta. */
+int lqcd_hmc_trajectory(lqcd_lattice* lat, double beta, double dt, size_t n, int drift,
+                        int scheme, uint64_t uniform, lqcd_hmc_stats* stats);
+
+/* ------------------------------------------------------ Wilson adjoint */
+/* Forward: S_W = beta * sum_{x, i<j} (1 - Re Tr P_ij(x) / 3), the action of
+ * src/simulation/state.rs:821-849. */
+int lqcd_wilson_forward(lqcd_lattice* lat, double beta, double* action);
+/* Backward: grad[site][mu][a] = dS_W / dw^a for U_mu(x) -> exp(i w^a T^a) U_mu(x);
+ * equals -(beta a / sqrt 6) * derivative_e (src/simulation/state.rs:1420-1448).
+ * Staples are recomputed, nothing is stored by the forward pass. */
+int lqcd_wilson_backward(lqcd_lattice* lat, double beta, double* grad, size_t n_links);
+
+/* ----------------------------------------------------------- Metropolis */
+typedef struct {
+    double prob_replace_mean;    /* MetropolisHastingsSweep::prob_replace_mean */
+    size_t number_replace_last;  /* MetropolisHastingsSweep::number_replace_last */
+} lqcd_mh_stats;
+
+/* One MetropolisHastingsSweep::next_element
+ * (src/simulation/monte_carlo/metropolis_hastings_sweep.rs:146-174) in
+ * checkerboard order: for mu = 0..D-1, parity 0 then 1, all links of that
+ * class at once.  proposals: [n_update][link][18] column-major 3x3 complex
+ * (already orthonormalized, applied as R_k...R_1 U); uniforms: one u64 per
+ * link.  Needs even extents (else LQCD_ERR_UNSUPPORTED). */
+int lqcd_metropolis_sweep(lqcd_lattice* lat, double beta, size_t n_update,
+                          const double* proposals, const uint64_t* uniforms,
+                          lqcd_mh_stats* stats);
+
+/* LatticeStateDefault::normalize_link_matrices (src/simulation/state.rs:754-756) */
+int lqcd_normalize_link_matrices(lqcd_lattice* lat);
+
+/* ------------------------------------------------------------ profiling */
+/* Kernel kinds for lqcd_profile_read (HIP events around each launch). */
+typedef enum {
+    LQCD_KERNEL_KICK_DRIFT = 0,
+    LQCD_KERNEL_KICK = 1,
+    LQCD_KERNEL_DRIFT = 2,
+    LQCD_KERNEL_PLAQUETTE = 3,
+    LQCD_KERNEL_EFIELD_ENERGY = 4,
+    LQCD_KERNEL_REUNIT = 5,
+    LQCD_KERNEL_METROPOLIS = 6,
+    LQCD_KERNEL_MOMENTA = 7,
+    LQCD_KERNEL_FORCE = 8
+} lqcd_kernel_kind;
+
+/* Record HIP events around every launch of the kinds in `mask` (bit = kind)
+ * on the context's stream; mask 0 stops recording.  Clears past records. */
+int lqcd_profile_enable(lqcd_ctx* ctx, unsigned mask);
+/* Synchronizes, then returns the number of recorded launches of `kind` and
+ * their summed duration in milliseconds. */
+int lqcd_profile_read(lqcd_ctx* ctx, int kind, size_t* count, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LQCD_H */

@@ -1,0 +1,587 @@
+// capi.cpp -- the extern "C" boundary of include/lqcd.h.
+//
+// Owns device memory and the per-context HIP stream, validates arguments with
+// the reference's error semantics, and sequences the kernels of one
+// integration step / trajectory / sweep.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/lqcd.h"
+#include "internal.hpp"
+
+using namespace lqcd;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t _e = (expr);                                                        \
+        if (_e != hipSuccess)                                                          \
+            return fail(_e == hipErrorOutOfMemory ? LQCD_ERR_OUT_OF_MEMORY : LQCD_ERR_DEVICE, \
+                        std::string(#expr) + ": " + hipGetErrorString(_e));            \
+    } while (0)
+
+#define CHECK_ARG(cond, msg)                                                           \
+    do {                                                                               \
+        if (!(cond)) return fail(LQCD_ERR_INVALID_ARGUMENT, msg);                      \
+    } while (0)
+
+struct DeviceGuard {
+    explicit DeviceGuard(int dev) { (void)hipSetDevice(dev); }
+};
+
+// Temporary device buffer freed on scope exit.
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
+int other_buffers(int cur, int* a, int* b)
+{
+    *a = (cur + 1) % 3;
+    *b = (cur + 2) % 3;
+    return 0;
+}
+
+int read_results(lqcd_lattice* L, int n)
+{
+    HIP_TRY(hipMemcpyAsync(L->host_result, L->result, sizeof(double) * n, hipMemcpyDeviceToHost,
+                           L->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    return LQCD_OK;
+}
+
+// H_U (beta * sum (1 - Re Tr P / 3)) of buffer U into host_result[2] and the
+// plaquette trace sum into host_result[0..1]; E energy into host_result[3].
+int hamiltonian_terms(lqcd_lattice* L, const void* U, bool with_e)
+{
+    HIP_TRY(launch_plaquette_sum(L, U, L->result));
+    if (with_e) HIP_TRY(launch_efield_energy(L, L->E, L->result + 3));
+    return read_results(L, with_e ? 4 : 3);
+}
+
+// n steps of the chosen scheme starting from U[cur]; *last gets the buffer
+// holding the final links.  U[cur] is left untouched.
+int integrate_steps(lqcd_lattice* L, double dt, size_t n, int drift, int scheme, int* last)
+{
+    int w1, w2;
+    other_buffers(L->cur, &w1, &w2);
+    int src = L->cur, dst = w1;
+    if (scheme == LQCD_SCHEME_REFERENCE) {
+        for (size_t k = 0; k < n; ++k) {
+            HIP_TRY(launch_kick_drift(L, L->U[src], L->U[dst], L->E, dt / 2.0, dt, drift));
+            HIP_TRY(launch_kick(L, L->U[dst], L->E, dt / 2.0));
+            src = dst;
+            dst = (dst == w1) ? w2 : w1;
+        }
+    } else {
+        for (size_t k = 0; k < n; ++k) {
+            HIP_TRY(launch_kick_drift(L, L->U[src], L->U[dst], L->E, k == 0 ? dt / 2.0 : dt, dt, drift));
+            src = dst;
+            dst = (dst == w1) ? w2 : w1;
+        }
+        HIP_TRY(launch_kick(L, L->U[src], L->E, dt / 2.0));
+    }
+    *last = src;
+    return LQCD_OK;
+}
+
+int check_links(const lqcd_lattice* L, size_t n_links)
+{
+    if (n_links != L->nlink)
+        return fail(LQCD_ERR_INCOMPATIBLE_SIZE, "size of lattice and data are incompatible: expected " +
+                                                    std::to_string(L->nlink) + " links, got " +
+                                                    std::to_string(n_links));
+    return LQCD_OK;
+}
+
+} // namespace
+
+namespace lqcd {
+
+void prof_begin(lqcd_ctx* ctx, int kind, lqcd_prof_event* ev)
+{
+    ev->kind = -1;
+    if (!ctx->profiling || !(ctx->profile_mask & (1u << kind))) return;
+    lqcd_prof_event e;
+    if (!ctx->event_pool.empty()) {
+        e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+    } else {
+        if (hipEventCreate(&e.start) != hipSuccess) return;
+        if (hipEventCreate(&e.stop) != hipSuccess) {
+            (void)hipEventDestroy(e.start);
+            return;
+        }
+    }
+    e.kind = kind;
+    (void)hipEventRecord(e.start, ctx->stream);
+    *ev = e;
+}
+
+void prof_end(lqcd_ctx* ctx, lqcd_prof_event* ev)
+{
+    if (ev->kind < 0) return;
+    (void)hipEventRecord(ev->stop, ctx->stream);
+    ctx->events.push_back(*ev);
+}
+
+} // namespace lqcd
+
+extern "C" {
+
+const char* lqcd_last_error(void) { return g_last_error.c_str(); }
+
+const char* lqcd_version(void) { return "lqcd-mi355x 0.1.0 (gfx950)"; }
+
+int lqcd_device_count(int* count)
+{
+    CHECK_ARG(count, "count is null");
+    HIP_TRY(hipGetDeviceCount(count));
+    return LQCD_OK;
+}
+
+int lqcd_ctx_create(int device, lqcd_ctx** out)
+{
+    CHECK_ARG(out, "out is null");
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    CHECK_ARG(device >= 0 && device < n, "device index out of range");
+    HIP_TRY(hipSetDevice(device));
+    lqcd_ctx* ctx = new lqcd_ctx;
+    ctx->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        return fail(LQCD_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    *out = ctx;
+    return LQCD_OK;
+}
+
+int lqcd_ctx_destroy(lqcd_ctx* ctx)
+{
+    if (!ctx) return LQCD_OK;
+    DeviceGuard dg(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& e : ctx->events) {
+        (void)hipEventDestroy(e.start);
+        (void)hipEventDestroy(e.stop);
+    }
+    for (auto& e : ctx->event_pool) {
+        (void)hipEventDestroy(e.start);
+        (void)hipEventDestroy(e.stop);
+    }
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return LQCD_OK;
+}
+
+int lqcd_ctx_synchronize(lqcd_ctx* ctx)
+{
+    CHECK_ARG(ctx, "ctx is null");
+    DeviceGuard dg(ctx->device);
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return LQCD_OK;
+}
+
+int lqcd_lattice_create(lqcd_ctx* ctx, int dim, const size_t* extent, double size, int precision,
+                        lqcd_lattice** out)
+{
+    CHECK_ARG(ctx && out, "null argument");
+    // LatticeCyclic::new, src/lattice.rs:190-201
+    if (dim == 0) return fail(LQCD_ERR_ZERO_DIMENSION, "the lattice has zero dimension");
+    if (size <= 0.0 || std::isnan(size) || std::isinf(size))
+        return fail(LQCD_ERR_NON_POSITIVE_SIZE, "the lattice spacing must be positive and finite");
+    CHECK_ARG(extent, "extent is null");
+    if (dim < 2 || dim > kMaxDim)
+        return fail(LQCD_ERR_UNSUPPORTED, "dimension must be 2, 3 or 4");
+    for (int i = 0; i < dim; ++i)
+        if (extent[i] < 2) return fail(LQCD_ERR_DIM_TOO_SMALL, "every extent must be at least 2");
+    CHECK_ARG(precision == LQCD_F64 || precision == LQCD_F32, "unknown precision");
+    DeviceGuard dg(ctx->device);
+
+    lqcd_lattice* L = new lqcd_lattice;
+    L->ctx = ctx;
+    L->size = size;
+    L->precision = precision;
+    L->elem = precision == LQCD_F32 ? 4 : 8;
+    L->g.dim = dim;
+    int64_t n = 1;
+    for (int i = 0; i < kMaxDim; ++i) {
+        L->g.extent[i] = 1;
+        L->g.stride[i] = 0;
+    }
+    for (int i = 0; i < dim; ++i) {
+        if (extent[i] > (size_t)(1 << 30)) {
+            delete L;
+            return fail(LQCD_ERR_INVALID_ARGUMENT, "extent too large");
+        }
+        L->g.extent[i] = (int)extent[i];
+        L->g.stride[i] = n;
+        n *= (int64_t)extent[i];
+        if (n > ((int64_t)1 << 40)) {
+            delete L;
+            return fail(LQCD_ERR_INVALID_ARGUMENT, "lattice too large");
+        }
+    }
+    L->g.nsite = n;
+    L->g.npad = (n + 255) / 256 * 256;
+    L->nlink = (size_t)n * (size_t)dim;
+    const size_t ubytes = (size_t)dim * 9 * (size_t)L->g.npad * 2 * L->elem;
+    const size_t ebytes = (size_t)dim * 8 * (size_t)L->g.npad * L->elem;
+    hipError_t e = hipSuccess;
+    for (int b = 0; b < 3 && e == hipSuccess; ++b) e = hipMalloc(&L->U[b], ubytes);
+    if (e == hipSuccess) e = hipMalloc(&L->E, ebytes);
+    if (e == hipSuccess)
+        e = hipMalloc((void**)&L->partials, sizeof(double) * kMaxReduceBlocks * kReduceSlots);
+    if (e == hipSuccess) e = hipMalloc((void**)&L->result, sizeof(double) * 16);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&L->host_result, sizeof(double) * 16, 0);
+    if (e == hipSuccess) e = hipMemsetAsync(L->U[0], 0, ubytes, ctx->stream);
+    if (e == hipSuccess) e = launch_links_cold(L, L->U[0]);
+    if (e == hipSuccess) e = launch_efield_zero(L, L->E);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        lqcd_lattice_destroy(L);
+        return fail(e == hipErrorOutOfMemory ? LQCD_ERR_OUT_OF_MEMORY : LQCD_ERR_DEVICE,
+                    std::string("lattice allocation: ") + hipGetErrorString(e));
+    }
+    *out = L;
+    return LQCD_OK;
+}
+
+int lqcd_lattice_destroy(lqcd_lattice* L)
+{
+    if (!L) return LQCD_OK;
+    DeviceGuard dg(L->ctx->device);
+    (void)hipStreamSynchronize(L->ctx->stream);
+    for (int b = 0; b < 3; ++b)
+        if (L->U[b]) (void)hipFree(L->U[b]);
+    if (L->E) (void)hipFree(L->E);
+    if (L->partials) (void)hipFree(L->partials);
+    if (L->result) (void)hipFree(L->result);
+    if (L->host_result) (void)hipHostFree(L->host_result);
+    if (L->mh_p) (void)hipFree(L->mh_p);
+    if (L->mh_acc) (void)hipFree(L->mh_acc);
+    delete L;
+    return LQCD_OK;
+}
+
+int lqcd_lattice_info(const lqcd_lattice* L, size_t* nsite, size_t* nlink)
+{
+    CHECK_ARG(L, "lattice is null");
+    if (nsite) *nsite = (size_t)L->g.nsite;
+    if (nlink) *nlink = L->nlink;
+    return LQCD_OK;
+}
+
+int lqcd_links_upload(lqcd_lattice* L, const double* links, size_t n_links)
+{
+    CHECK_ARG(L && links, "null argument");
+    if (int rc = check_links(L, n_links)) return rc;
+    DeviceGuard dg(L->ctx->device);
+    DevBuf st;
+    HIP_TRY(hipMalloc(&st.p, sizeof(double) * 18 * L->nlink));
+    HIP_TRY(hipMemcpyAsync(st.p, links, sizeof(double) * 18 * L->nlink, hipMemcpyHostToDevice,
+                           L->ctx->stream));
+    HIP_TRY(launch_links_from_aos(L, (const double*)st.p, L->U[L->cur]));
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    return LQCD_OK;
+}
+
+int lqcd_links_download(lqcd_lattice* L, double* links, size_t n_links)
+{
+    CHECK_ARG(L && links, "null argument");
+    if (int rc = check_links(L, n_links)) return rc;
+    DeviceGuard dg(L->ctx->device);
+    DevBuf st;
+    HIP_TRY(hipMalloc(&st.p, sizeof(double) * 18 * L->nlink));
+    HIP_TRY(launch_links_to_aos(L, L->U[L->cur], (double*)st.p));
+    HIP_TRY(hipMemcpyAsync(links, st.p, sizeof(double) * 18 * L->nlink, hipMemcpyDeviceToHost,
+                           L->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    return LQCD_OK;
+}
+
+int lqcd_links_cold(lqcd_lattice* L)
+{
+    CHECK_ARG(L, "lattice is null");
+    DeviceGuard dg(L->ctx->device);
+    HIP_TRY(launch_links_cold(L, L->U[L->cur]));
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    return LQCD_OK;
+}
+
+int lqcd_efield_upload(lqcd_lattice* L, const double* efield, size_t n_links)
+{
+    CHECK_ARG(L && efield, "null argument");
+    if (int rc = check_links(L, n_links)) return rc;
+    DeviceGuard dg(L->ctx->device);
+    DevBuf st;
+    HIP_TRY(hipMalloc(&st.p, sizeof(double) * 8 * L->nlink));
+    HIP_TRY(hipMemcpyAsync(st.p, efield, sizeof(double) * 8 * L->nlink, hipMemcpyHostToDevice,
+                           L->ctx->stream));
+    HIP_TRY(launch_efield_from_aos(L, (const double*)st.p, L->E));
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    return LQCD_OK;
+}
+
+int lqcd_efield_download(lqcd_lattice* L, double* efield, size_t n_links)
+{
+    CHECK_ARG(L && efield, "null argument");
+    if (int rc = check_links(L, n_links)) return rc;
+    DeviceGuard dg(L->ctx->device);
+    DevBuf st;
+    HIP_TRY(hipMalloc(&st.p, sizeof(double) * 8 * L->nlink));
+    HIP_TRY(launch_efield_to_aos(L, L->E, (double*)st.p));
+    HIP_TRY(hipMemcpyAsync(efield, st.p, sizeof(double) * 8 * L->nlink, hipMemcpyDeviceToHost,
+                           L->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    return LQCD_OK;
+}
+
+int lqcd_efield_cold(lqcd_lattice* L)
+{
+    CHECK_ARG(L, "lattice is null");
+    DeviceGuard dg(L->ctx->device);
+    HIP_TRY(launch_efield_zero(L, L->E));
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    return LQCD_OK;
+}
+
+int lqcd_efield_random(lqcd_lattice* L, uint64_t seed, uint64_t counter, double sigma)
+{
+    CHECK_ARG(L, "lattice is null");
+    CHECK_ARG(sigma >= 0.0 && std::isfinite(sigma), "sigma must be finite and >= 0");
+    DeviceGuard dg(L->ctx->device);
+    HIP_TRY(launch_device_momenta(L, L->E, seed, counter, sigma));
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    return LQCD_OK;
+}
+
+int lqcd_average_trace_plaquette(lqcd_lattice* L, double out[2])
+{
+    CHECK_ARG(L && out, "null argument");
+    DeviceGuard dg(L->ctx->device);
+    if (int rc = hamiltonian_terms(L, L->U[L->cur], false)) return rc;
+    const double nplaq = (double)L->g.nsite * (double)(L->g.dim * (L->g.dim - 1) / 2);
+    out[0] = L->host_result[0] / nplaq;
+    out[1] = L->host_result[1] / nplaq;
+    return LQCD_OK;
+}
+
+int lqcd_hamiltonian_links(lqcd_lattice* L, double beta, double* out)
+{
+    CHECK_ARG(L && out, "null argument");
+    DeviceGuard dg(L->ctx->device);
+    if (int rc = hamiltonian_terms(L, L->U[L->cur], false)) return rc;
+    *out = L->host_result[2] * beta;
+    return LQCD_OK;
+}
+
+int lqcd_hamiltonian_efield(lqcd_lattice* L, double beta, double* out)
+{
+    CHECK_ARG(L && out, "null argument");
+    DeviceGuard dg(L->ctx->device);
+    HIP_TRY(launch_efield_energy(L, L->E, L->result + 3));
+    HIP_TRY(hipMemcpyAsync(L->host_result + 3, L->result + 3, sizeof(double), hipMemcpyDeviceToHost,
+                           L->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    *out = L->host_result[3] * beta;
+    return LQCD_OK;
+}
+
+static int force_to_host(lqcd_lattice* L, double* out, size_t n_links, double scale)
+{
+    CHECK_ARG(L && out, "null argument");
+    if (int rc = check_links(L, n_links)) return rc;
+    DeviceGuard dg(L->ctx->device);
+    DevBuf st;
+    HIP_TRY(hipMalloc(&st.p, sizeof(double) * 8 * L->nlink));
+    HIP_TRY(launch_force_store(L, L->U[L->cur], (double*)st.p, scale));
+    HIP_TRY(hipMemcpyAsync(out, st.p, sizeof(double) * 8 * L->nlink, hipMemcpyDeviceToHost,
+                           L->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    return LQCD_OK;
+}
+
+int lqcd_derivative_e(lqcd_lattice* L, double* out, size_t n_links)
+{
+    return force_to_host(L, out, n_links, 1.0);
+}
+
+int lqcd_integrate_efield(lqcd_lattice* L, double dt)
+{
+    CHECK_ARG(L, "lattice is null");
+    CHECK_ARG(std::isfinite(dt), "dt must be finite");
+    DeviceGuard dg(L->ctx->device);
+    HIP_TRY(launch_kick(L, L->U[L->cur], L->E, dt));
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    return LQCD_OK;
+}
+
+int lqcd_integrate_link(lqcd_lattice* L, double dt, int drift)
+{
+    CHECK_ARG(L, "lattice is null");
+    CHECK_ARG(std::isfinite(dt), "dt must be finite");
+    CHECK_ARG(drift == LQCD_DRIFT_ADDITIVE || drift == LQCD_DRIFT_EXP, "unknown drift mode");
+    DeviceGuard dg(L->ctx->device);
+    int w1, w2;
+    other_buffers(L->cur, &w1, &w2);
+    HIP_TRY(launch_drift(L, L->U[L->cur], L->U[w1], L->E, dt, drift));
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    L->cur = w1;
+    return LQCD_OK;
+}
+
+int lqcd_simulate_symplectic_n(lqcd_lattice* L, double dt, size_t n, int drift, int scheme)
+{
+    CHECK_ARG(L, "lattice is null");
+    if (n == 0) return fail(LQCD_ERR_ZERO_INTEGRATION, "no integration steps");
+    CHECK_ARG(std::isfinite(dt), "dt must be finite");
+    CHECK_ARG(drift == LQCD_DRIFT_ADDITIVE || drift == LQCD_DRIFT_EXP, "unknown drift mode");
+    CHECK_ARG(scheme == LQCD_SCHEME_REFERENCE || scheme == LQCD_SCHEME_LEAPFROG, "unknown scheme");
+    DeviceGuard dg(L->ctx->device);
+    int last = L->cur;
+    if (int rc = integrate_steps(L, dt, n, drift, scheme, &last)) return rc;
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    L->cur = last;
+    return LQCD_OK;
+}
+
+int lqcd_hmc_trajectory(lqcd_lattice* L, double beta, double dt, size_t n, int drift, int scheme,
+                        uint64_t uniform, lqcd_hmc_stats* stats)
+{
+    CHECK_ARG(L, "lattice is null");
+    if (n == 0) return fail(LQCD_ERR_ZERO_INTEGRATION, "no integration steps");
+    CHECK_ARG(std::isfinite(dt) && std::isfinite(beta), "dt and beta must be finite");
+    CHECK_ARG(drift == LQCD_DRIFT_ADDITIVE || drift == LQCD_DRIFT_EXP, "unknown drift mode");
+    CHECK_ARG(scheme == LQCD_SCHEME_REFERENCE || scheme == LQCD_SCHEME_LEAPFROG, "unknown scheme");
+    DeviceGuard dg(L->ctx->device);
+    if (int rc = hamiltonian_terms(L, L->U[L->cur], true)) return rc;
+    const double h_old = beta * L->host_result[2] + beta * L->host_result[3];
+    int last = L->cur;
+    if (int rc = integrate_steps(L, dt, n, drift, scheme, &last)) return rc;
+    if (int rc = hamiltonian_terms(L, L->U[last], true)) return rc;
+    const double h_new = beta * L->host_result[2] + beta * L->host_result[3];
+    // probability_of_replacement, hybrid_monte_carlo.rs:584-589 (clamped twice)
+    double p = std::fmax(std::fmin(std::exp(h_old - h_new), 1.0), 0.0);
+    p = std::fmax(std::fmin(p, 1.0), 0.0);
+    bool accept;
+    if (p >= 1.0) {
+        accept = true;
+    } else {
+        const uint64_t p_int = (uint64_t)(p * 18446744073709551616.0);
+        accept = uniform < p_int;
+    }
+    if (accept) L->cur = last;
+    if (stats) {
+        stats->h_old = h_old;
+        stats->h_new = h_new;
+        stats->delta_h = h_new - h_old;
+        stats->probability = p;
+        stats->accepted = accept ? 1 : 0;
+    }
+    return LQCD_OK;
+}
+
+int lqcd_wilson_forward(lqcd_lattice* L, double beta, double* action)
+{
+    return lqcd_hamiltonian_links(L, beta, action);
+}
+
+int lqcd_wilson_backward(lqcd_lattice* L, double beta, double* grad, size_t n_links)
+{
+    CHECK_ARG(std::isfinite(beta), "beta must be finite");
+    CHECK_ARG(L, "lattice is null");
+    return force_to_host(L, grad, n_links, -(beta * L->size) / std::sqrt(6.0));
+}
+
+int lqcd_metropolis_sweep(lqcd_lattice* L, double beta, size_t n_update, const double* proposals,
+                          const uint64_t* uniforms, lqcd_mh_stats* stats)
+{
+    CHECK_ARG(L && proposals && uniforms, "null argument");
+    CHECK_ARG(n_update > 0, "n_update must be positive");
+    CHECK_ARG(std::isfinite(beta), "beta must be finite");
+    for (int i = 0; i < L->g.dim; ++i)
+        if (L->g.extent[i] % 2)
+            return fail(LQCD_ERR_UNSUPPORTED, "checkerboard sweep needs even extents");
+    DeviceGuard dg(L->ctx->device);
+    if (!L->mh_p) HIP_TRY(hipMalloc((void**)&L->mh_p, sizeof(double) * L->nlink));
+    if (!L->mh_acc) HIP_TRY(hipMalloc((void**)&L->mh_acc, sizeof(uint32_t) * L->nlink));
+    DevBuf dp, du;
+    const size_t pbytes = sizeof(double) * 18 * L->nlink * n_update;
+    HIP_TRY(hipMalloc(&dp.p, pbytes));
+    HIP_TRY(hipMalloc(&du.p, sizeof(uint64_t) * L->nlink));
+    HIP_TRY(hipMemcpyAsync(dp.p, proposals, pbytes, hipMemcpyHostToDevice, L->ctx->stream));
+    HIP_TRY(hipMemcpyAsync(du.p, uniforms, sizeof(uint64_t) * L->nlink, hipMemcpyHostToDevice,
+                           L->ctx->stream));
+    for (int mu = 0; mu < L->g.dim; ++mu)
+        for (int par = 0; par < 2; ++par)
+            HIP_TRY(launch_metropolis_class(L, L->U[L->cur], mu, par, beta, n_update,
+                                            (const double*)dp.p, (const uint64_t*)du.p));
+    HIP_TRY(launch_mh_stats(L, L->result));
+    if (int rc = read_results(L, 2)) return rc;
+    if (stats) {
+        stats->prob_replace_mean = L->host_result[0] / (double)L->nlink;
+        stats->number_replace_last = (size_t)L->host_result[1];
+    }
+    return LQCD_OK;
+}
+
+int lqcd_normalize_link_matrices(lqcd_lattice* L)
+{
+    CHECK_ARG(L, "lattice is null");
+    DeviceGuard dg(L->ctx->device);
+    HIP_TRY(launch_reunitarize(L, L->U[L->cur]));
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    return LQCD_OK;
+}
+
+int lqcd_profile_enable(lqcd_ctx* ctx, unsigned mask)
+{
+    CHECK_ARG(ctx, "ctx is null");
+    DeviceGuard dg(ctx->device);
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (auto& e : ctx->events) ctx->event_pool.push_back(e);
+    ctx->events.clear();
+    ctx->profile_mask = mask;
+    ctx->profiling = mask != 0;
+    return LQCD_OK;
+}
+
+int lqcd_profile_read(lqcd_ctx* ctx, int kind, size_t* count, double* total_ms)
+{
+    CHECK_ARG(ctx && count && total_ms, "null argument");
+    DeviceGuard dg(ctx->device);
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    size_t c = 0;
+    double t = 0.0;
+    for (auto& e : ctx->events) {
+        if (e.kind != kind) continue;
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e.start, e.stop));
+        t += ms;
+        ++c;
+    }
+    *count = c;
+    *total_ms = t;
+    return LQCD_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,106 @@
+// internal.hpp -- host-side objects behind the C ABI of include/lqcd.h and the
+// kernel launchers shared between translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "geometry.hpp"
+
+struct lqcd_prof_event {
+    int kind;
+    hipEvent_t start, stop;
+};
+
+struct lqcd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool profiling = false;
+    std::vector<lqcd_prof_event> events;     // recorded launches of the profiled kinds
+    std::vector<lqcd_prof_event> event_pool; // reusable event pairs
+    unsigned profile_mask = 0;
+};
+
+struct lqcd_lattice {
+    lqcd_ctx* ctx = nullptr;
+    lqcd::Geom g{};
+    double size = 1.0;       // lattice spacing a
+    int precision = 0;       // LQCD_F64 / LQCD_F32
+    size_t nlink = 0;
+    size_t elem = 8;         // bytes of one stored real
+    void* U[3] = {nullptr, nullptr, nullptr};
+    int cur = 0;             // buffer holding the current links
+    void* E = nullptr;
+    double* partials = nullptr;  // per-block reduction scratch
+    double* result = nullptr;    // device reduction results
+    double* host_result = nullptr; // pinned mirror of result
+    double* mh_p = nullptr;        // per-link Metropolis probabilities
+    uint32_t* mh_acc = nullptr;    // per-link Metropolis accept flags
+    int nblocks_reduce = 0;
+};
+
+namespace lqcd {
+
+constexpr int kBlock = 256;
+constexpr int kReduceSlots = 4;   // doubles per block partial
+constexpr int kMaxReduceBlocks = 4096;
+
+enum ProfileKind {
+    PROF_KICK_DRIFT = 0,
+    PROF_KICK = 1,
+    PROF_DRIFT = 2,
+    PROF_PLAQUETTE = 3,
+    PROF_EFIELD_ENERGY = 4,
+    PROF_REUNIT = 5,
+    PROF_METROPOLIS = 6,
+    PROF_MOMENTA = 7,
+    PROF_FORCE_STORE = 8,
+    PROF_NKINDS = 9
+};
+
+// ---- fields (k_fields.hip) ----
+hipError_t launch_links_from_aos(const lqcd_lattice* L, const double* d_aos, void* U);
+hipError_t launch_links_to_aos(const lqcd_lattice* L, const void* U, double* d_aos);
+hipError_t launch_efield_from_aos(const lqcd_lattice* L, const double* d_aos, void* E);
+hipError_t launch_efield_to_aos(const lqcd_lattice* L, const void* E, double* d_aos);
+hipError_t launch_links_cold(const lqcd_lattice* L, void* U);
+hipError_t launch_efield_zero(const lqcd_lattice* L, void* E);
+hipError_t launch_device_momenta(const lqcd_lattice* L, void* E, uint64_t seed, uint64_t counter,
+                                 double sigma);
+hipError_t launch_reunitarize(const lqcd_lattice* L, void* U);
+
+// ---- observables (k_observables.hip) ----
+// out[0..1] = sum Tr P (complex), out[2] = sum (1 - Re Tr P / 3)
+hipError_t launch_plaquette_sum(const lqcd_lattice* L, const void* U, double* d_out3);
+// out[0] = sum_a e_a^2 / 2 over all links
+hipError_t launch_efield_energy(const lqcd_lattice* L, const void* E, double* d_out1);
+// out[0] = sum p, out[1] = number accepted
+hipError_t launch_mh_stats(const lqcd_lattice* L, double* d_out2);
+
+// ---- molecular dynamics (k_md.hip) ----
+enum DriftMode { DRIFT_ADDITIVE = 0, DRIFT_EXP = 1 };
+// out[(s*D+mu)*8+a] = scale * derivative_e^a
+hipError_t launch_force_store(const lqcd_lattice* L, const void* U, double* d_out, double scale);
+// E += kick * derivative_e(U)
+hipError_t launch_kick(const lqcd_lattice* L, const void* U, void* E, double kick);
+// E += kick * derivative_e(U_in); U_out = drift(U_in, E, dt)
+hipError_t launch_kick_drift(const lqcd_lattice* L, const void* U_in, void* U_out, void* E, double kick,
+                             double dt, int drift_mode);
+// U_out = drift(U_in, E, dt)
+hipError_t launch_drift(const lqcd_lattice* L, const void* U_in, void* U_out, const void* E, double dt,
+                        int drift_mode);
+
+// ---- Metropolis (k_mc.hip) ----
+hipError_t launch_metropolis_class(const lqcd_lattice* L, void* U, int mu, int parity, double beta,
+                                   size_t n_update, const double* d_proposals,
+                                   const uint64_t* d_uniforms);
+
+// profiling hooks (capi.cpp)
+void prof_begin(lqcd_ctx* ctx, int kind, lqcd_prof_event* ev);
+void prof_end(lqcd_ctx* ctx, lqcd_prof_event* ev);
+
+} // namespace lqcd

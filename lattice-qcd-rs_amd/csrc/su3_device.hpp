@@ -1,0 +1,304 @@
+// su3_device.hpp -- SU(3)/su(3) arithmetic for gfx950 kernels.
+//
+// Matrices are row-major 3x3 complex held in registers (e[r*3+c]).  Products
+// take compile-time adjoint flags so a^dagger * b^dagger etc. cost the same as
+// a * b: the conjugations fold into the sign modifiers of v_fma.  Everything
+// is templated on the arithmetic type T (float or double).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace lqcd {
+
+template <typename T>
+struct cplx {
+    T re, im;
+};
+
+template <typename T>
+struct mat3 {
+    cplx<T> e[9];
+    __device__ __forceinline__ cplx<T>& operator()(int r, int c) { return e[r * 3 + c]; }
+    __device__ __forceinline__ const cplx<T>& operator()(int r, int c) const { return e[r * 3 + c]; }
+};
+
+template <typename T>
+__device__ __forceinline__ void set_zero(mat3<T>& a)
+{
+#pragma unroll
+    for (int k = 0; k < 9; ++k) a.e[k] = {T(0), T(0)};
+}
+
+template <typename T>
+__device__ __forceinline__ void set_identity(mat3<T>& a)
+{
+    set_zero(a);
+    a.e[0].re = T(1);
+    a.e[4].re = T(1);
+    a.e[8].re = T(1);
+}
+
+// element (i, k) of a or of a^dagger
+template <bool DAG, typename T>
+__device__ __forceinline__ cplx<T> elem(const mat3<T>& a, int i, int k)
+{
+    if (DAG) {
+        const cplx<T> z = a.e[k * 3 + i];
+        return {z.re, -z.im};
+    }
+    return a.e[i * 3 + k];
+}
+
+// c = op(a) * op(b); c may not alias a or b
+template <bool DA, bool DB, typename T>
+__device__ __forceinline__ void mul(const mat3<T>& a, const mat3<T>& b, mat3<T>& c)
+{
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            T re = T(0), im = T(0);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const cplx<T> x = elem<DA>(a, i, k);
+                const cplx<T> y = elem<DB>(b, k, j);
+                re = fma(x.re, y.re, re);
+                re = fma(-x.im, y.im, re);
+                im = fma(x.re, y.im, im);
+                im = fma(x.im, y.re, im);
+            }
+            c.e[i * 3 + j] = {re, im};
+        }
+}
+
+// c += op(a) * op(b)
+template <bool DA, bool DB, typename T>
+__device__ __forceinline__ void mul_add(const mat3<T>& a, const mat3<T>& b, mat3<T>& c)
+{
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            T re = c.e[i * 3 + j].re, im = c.e[i * 3 + j].im;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const cplx<T> x = elem<DA>(a, i, k);
+                const cplx<T> y = elem<DB>(b, k, j);
+                re = fma(x.re, y.re, re);
+                re = fma(-x.im, y.im, re);
+                im = fma(x.re, y.im, im);
+                im = fma(x.im, y.re, im);
+            }
+            c.e[i * 3 + j] = {re, im};
+        }
+}
+
+// Re Tr(op(a) * op(b)) without forming the product
+template <bool DA, bool DB, typename T>
+__device__ __forceinline__ cplx<T> trace_mul(const mat3<T>& a, const mat3<T>& b)
+{
+    T re = T(0), im = T(0);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const cplx<T> x = elem<DA>(a, i, k);
+            const cplx<T> y = elem<DB>(b, k, i);
+            re = fma(x.re, y.re, re);
+            re = fma(-x.im, y.im, re);
+            im = fma(x.re, y.im, im);
+            im = fma(x.im, y.re, im);
+        }
+    return {re, im};
+}
+
+template <typename T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b)
+{
+    return {fma(a.re, b.re, -a.im * b.im), fma(a.re, b.im, a.im * b.re)};
+}
+
+template <typename T>
+__device__ __forceinline__ cplx<T> det(const mat3<T>& m)
+{
+    const cplx<T> a = cmul(m.e[4], m.e[8]), b = cmul(m.e[7], m.e[5]);
+    const cplx<T> c = cmul(m.e[3], m.e[8]), d = cmul(m.e[6], m.e[5]);
+    const cplx<T> f = cmul(m.e[3], m.e[7]), g = cmul(m.e[6], m.e[4]);
+    const cplx<T> m0 = {a.re - b.re, a.im - b.im};
+    const cplx<T> m1 = {c.re - d.re, c.im - d.im};
+    const cplx<T> m2 = {f.re - g.re, f.im - g.im};
+    const cplx<T> t0 = cmul(m.e[0], m0), t1 = cmul(m.e[1], m1), t2 = cmul(m.e[2], m2);
+    return {t0.re - t1.re + t2.re, t0.im - t1.im + t2.im};
+}
+
+// su(3) element sum_a v_a T^a (T^a = Gell-Mann / 2) as a hermitian matrix
+template <typename T>
+__device__ __forceinline__ void adjoint_to_matrix(const T* v, mat3<T>& m)
+{
+    const T h = T(0.5);
+    const T r3 = T(0.28867513459481288225); // 1 / (2 sqrt 3)
+    m.e[0] = {h * v[2] + r3 * v[7], T(0)};
+    m.e[4] = {-h * v[2] + r3 * v[7], T(0)};
+    m.e[8] = {T(-2) * r3 * v[7], T(0)};
+    m.e[1] = {h * v[0], -h * v[1]};
+    m.e[3] = {h * v[0], h * v[1]};
+    m.e[2] = {h * v[3], -h * v[4]};
+    m.e[6] = {h * v[3], h * v[4]};
+    m.e[5] = {h * v[5], -h * v[6]};
+    m.e[7] = {h * v[5], h * v[6]};
+}
+
+// out_a = Im Tr(T^a M), closed form of the eight generator projections
+template <typename T>
+__device__ __forceinline__ void im_trace_generators(const mat3<T>& M, T* out)
+{
+    const T h = T(0.5);
+    const T r3 = T(0.28867513459481288225);
+    out[0] = h * (M.e[1].im + M.e[3].im);
+    out[1] = h * (M.e[1].re - M.e[3].re);
+    out[2] = h * (M.e[0].im - M.e[4].im);
+    out[3] = h * (M.e[2].im + M.e[6].im);
+    out[4] = h * (M.e[2].re - M.e[6].re);
+    out[5] = h * (M.e[5].im + M.e[7].im);
+    out[6] = h * (M.e[5].re - M.e[7].re);
+    out[7] = r3 * (M.e[0].im + M.e[4].im - T(2) * M.e[8].im);
+}
+
+// 1/k! for k = 0..25
+__device__ __constant__ const double kInvFactorial[26] = {
+    1.0,
+    1.0,
+    0.5,
+    0.16666666666666666,
+    0.041666666666666664,
+    0.008333333333333333,
+    0.001388888888888889,
+    0.0001984126984126984,
+    2.48015873015873e-05,
+    2.7557319223985893e-06,
+    2.755731922398589e-07,
+    2.505210838544172e-08,
+    2.08767569878681e-09,
+    1.6059043836821613e-10,
+    1.1470745597729725e-11,
+    7.647163731819816e-13,
+    4.779477332387385e-14,
+    2.8114572543455206e-15,
+    1.5619206968586225e-16,
+    8.22063524662433e-18,
+    4.110317623312165e-19,
+    1.9572941063391263e-20,
+    8.896791392450574e-22,
+    3.8681701706306835e-23,
+    1.6117375710961184e-24,
+    6.446950284384474e-26,
+};
+
+// exp(i X) for hermitian traceless X given by its adjoint coordinates v
+// (X = sum v_a T^a) by the Cayley-Hamilton recursion over 1/k! of
+// su3_exp_i, src/su3.rs:832-855: with d = i det X and t = -Tr(X^2)/2,
+// q0 <- 1/k! + d q2, q1 <- i (q0 - t q2), q2 <- i q1 for k = 24 .. 0,
+// exp(iX) = q0 + q1 X + q2 X^2.
+template <typename T>
+__device__ __forceinline__ void exp_i(const T* v, mat3<T>& out)
+{
+    mat3<T> X, X2;
+    adjoint_to_matrix(v, X);
+    mul<false, false>(X, X, X2);
+    T ss = T(0);
+#pragma unroll
+    for (int a = 0; a < 8; ++a) ss = fma(v[a], v[a], ss);
+    const T t = T(-0.25) * ss;
+    const cplx<T> dx = det(X);
+    const cplx<T> d = {-dx.im, dx.re};
+    cplx<T> q0 = {T(kInvFactorial[25]), T(0)}, q1 = {T(0), T(0)}, q2 = {T(0), T(0)};
+#pragma unroll
+    for (int k = 24; k >= 0; --k) {
+        const cplx<T> dq2 = cmul(d, q2);
+        const cplx<T> q0n = {T(kInvFactorial[k]) + dq2.re, dq2.im};
+        const cplx<T> w = {fma(-t, q2.re, q0.re), fma(-t, q2.im, q0.im)};
+        const cplx<T> q1n = {-w.im, w.re};
+        const cplx<T> q2n = {-q1.im, q1.re};
+        q0 = q0n;
+        q1 = q1n;
+        q2 = q2n;
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const cplx<T> x = X.e[k], y = X2.e[k];
+        T re = fma(q1.re, x.re, -q1.im * x.im);
+        T im = fma(q1.re, x.im, q1.im * x.re);
+        re = fma(q2.re, y.re, re);
+        re = fma(-q2.im, y.im, re);
+        im = fma(q2.re, y.im, im);
+        im = fma(q2.im, y.re, im);
+        out.e[k] = {re, im};
+    }
+    out.e[0].re += q0.re; out.e[0].im += q0.im;
+    out.e[4].re += q0.re; out.e[4].im += q0.im;
+    out.e[8].re += q0.re; out.e[8].im += q0.im;
+}
+
+// column Gram-Schmidt of orthonormalize_matrix, src/su3.rs:279-303:
+// c0 /= |c0| (unless |c0| <= eps), c1 -= c0 (c0^dagger c1), c1 /= |c1|
+// (same rule), c2 = conj(c0) x conj(c1).  Double arithmetic only.
+__device__ __forceinline__ void orthonormalize(mat3<double>& m)
+{
+    const double eps = 2.220446049250313e-16;
+    cplx<double> a[3], b[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        a[i] = m.e[i * 3 + 0];
+        b[i] = m.e[i * 3 + 1];
+    }
+    double n0 = sqrt(a[0].re * a[0].re + a[0].im * a[0].im + a[1].re * a[1].re +
+                     a[1].im * a[1].im + a[2].re * a[2].re + a[2].im * a[2].im);
+    if (n0 > eps) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) a[i] = {a[i].re / n0, a[i].im / n0};
+    }
+    // s = sum conj(a_i) b_i
+    cplx<double> s = {0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        s.re += a[i].re * b[i].re + a[i].im * b[i].im;
+        s.im += a[i].re * b[i].im - a[i].im * b[i].re;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const cplx<double> p = cmul(a[i], s);
+        b[i] = {b[i].re - p.re, b[i].im - p.im};
+    }
+    double n1 = sqrt(b[0].re * b[0].re + b[0].im * b[0].im + b[1].re * b[1].re +
+                     b[1].im * b[1].im + b[2].re * b[2].re + b[2].im * b[2].im);
+    if (n1 > eps) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) b[i] = {b[i].re / n1, b[i].im / n1};
+    }
+    const cplx<double> ca[3] = {{a[0].re, -a[0].im}, {a[1].re, -a[1].im}, {a[2].re, -a[2].im}};
+    const cplx<double> cb[3] = {{b[0].re, -b[0].im}, {b[1].re, -b[1].im}, {b[2].re, -b[2].im}};
+    cplx<double> c[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const int j = (i + 1) % 3, k = (i + 2) % 3;
+        const cplx<double> p = cmul(ca[j], cb[k]);
+        const cplx<double> q = cmul(ca[k], cb[j]);
+        c[i] = {p.re - q.re, p.im - q.im};
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        m.e[i * 3 + 0] = a[i];
+        m.e[i * 3 + 1] = b[i];
+        m.e[i * 3 + 2] = c[i];
+    }
+}
+
+template <typename T, typename U>
+__device__ __forceinline__ mat3<T> convert(const mat3<U>& m)
+{
+    mat3<T> r;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) r.e[k] = {T(m.e[k].re), T(m.e[k].im)};
+    return r;
+}
+
+} // namespace lqcd

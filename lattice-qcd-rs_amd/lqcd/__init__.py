@@ -1,0 +1,489 @@
+"""Host-side mirror of lattice-qcd-rs's state / integrator / Monte Carlo traits
+over the gfx950 library (include/lqcd.h).
+
+The reference's API is Rust generics over ``const D``; here the dimension is a
+runtime argument and the fields live in HBM behind a lattice handle:
+
+=====================================  ==========================================
+reference (lattice-qcd-rs)              here
+=====================================  ==========================================
+``LatticeCyclic::<D>::new(size, dim)``  ``LatticeCyclic(size, dim, d)``
+``LatticeStateDefault<D>``              ``LatticeStateDefault`` (device resident)
+``LatticeStateEFSyncDefault``           ``LatticeStateEFSyncDefault``
+``SymplecticEulerRayon``                ``SymplecticEulerGpu``
+``HybridMonteCarloDiagnostic``          ``HybridMonteCarloDiagnostic``
+``MetropolisHastingsSweep``             ``MetropolisHastingsSweep`` (checkerboard)
+=====================================  ==========================================
+
+Host arrays use the reference's layouts: links ``(nsite*D, 18)`` float64,
+column-major 3x3 complex, link index ``site*D + mu``; E ``(nsite*D, 8)``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from typing import Optional, Sequence, Union
+
+import numpy as np
+
+from . import _ffi as F
+from ._ffi import (DRIFT_ADDITIVE, DRIFT_EXP, F32, F64, SCHEME_LEAPFROG, SCHEME_REFERENCE,
+                   LqcdError)
+
+__all__ = [
+    "Context", "LatticeCyclic", "LatticeStateDefault", "LatticeStateEFSyncDefault",
+    "SymplecticEulerGpu", "HybridMonteCarloDiagnostic", "MetropolisHastingsSweep",
+    "LatticeInitializationError", "StateInitializationError", "MultiIntegrationError",
+    "LqcdError", "F32", "F64", "DRIFT_ADDITIVE", "DRIFT_EXP", "SCHEME_REFERENCE",
+    "SCHEME_LEAPFROG",
+]
+
+
+class LatticeInitializationError(ValueError):
+    """src/error.rs:219-241 (ZeroDimension, NonPositiveSize, DimTooSmall)."""
+
+
+class StateInitializationError(ValueError):
+    """src/error.rs:124-171 (IncompatibleSize ...)."""
+
+
+class MultiIntegrationError(RuntimeError):
+    """src/error.rs:93-118 (ZeroIntegration, IntegrationError)."""
+
+
+def _raise(rc: int) -> None:
+    if rc == F.OK:
+        return
+    msg = F.lib().lqcd_last_error().decode(errors="replace")
+    if rc in (F.ERR_ZERO_DIMENSION, F.ERR_NON_POSITIVE_SIZE, F.ERR_DIM_TOO_SMALL):
+        raise LatticeInitializationError(msg)
+    if rc == F.ERR_INCOMPATIBLE_SIZE:
+        raise StateInitializationError(msg)
+    if rc == F.ERR_ZERO_INTEGRATION:
+        raise MultiIntegrationError(msg)
+    raise LqcdError(rc, msg)
+
+
+class Context:
+    """One HIP stream on one device (lqcd_ctx)."""
+
+    _default = {}
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _raise(F.lib().lqcd_ctx_create(device, C.byref(h)))
+        self.handle = h
+        self.device = device
+
+    @classmethod
+    def default(cls, device: int = 0) -> "Context":
+        if device not in cls._default:
+            cls._default[device] = Context(device)
+        return cls._default[device]
+
+    def synchronize(self) -> None:
+        _raise(F.lib().lqcd_ctx_synchronize(self.handle))
+
+    def profile_enable(self, mask: int) -> None:
+        _raise(F.lib().lqcd_profile_enable(self.handle, mask))
+
+    def profile_read(self, kind: int):
+        n = C.c_size_t(0)
+        t = C.c_double(0)
+        _raise(F.lib().lqcd_profile_read(self.handle, kind, C.byref(n), C.byref(t)))
+        return int(n.value), float(t.value)
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                F.lib().lqcd_ctx_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class LatticeCyclic:
+    """``LatticeCyclic::<D>::new(size, dim)`` (src/lattice.rs:190-201).
+
+    ``dim`` is the number of points per direction; a sequence gives one extent
+    per direction (the reference is hypercubic: all equal).
+    """
+
+    def __init__(self, size: float, dim: Union[int, Sequence[int]], d: int = 4):
+        if d == 0:
+            raise LatticeInitializationError("the lattice has zero dimension")
+        if not (size > 0) or math.isinf(size):
+            raise LatticeInitializationError("the lattice spacing must be positive and finite")
+        ext = [int(dim)] * d if np.isscalar(dim) else [int(x) for x in dim]
+        if len(ext) != d:
+            raise ValueError("one extent per direction")
+        if any(x < 2 for x in ext):
+            raise LatticeInitializationError("every extent must be at least 2")
+        self._size = float(size)
+        self._extent = ext
+        self._d = d
+
+    def dim_st(self) -> int:
+        return self._d
+
+    def dim(self) -> int:
+        return self._extent[0]
+
+    def extents(self):
+        return list(self._extent)
+
+    def size(self) -> float:
+        return self._size
+
+    def number_of_points(self) -> int:
+        return int(np.prod(self._extent))
+
+    def number_of_canonical_links_space(self) -> int:
+        return self.number_of_points() * self._d
+
+
+class _DeviceLattice:
+    """Owner of one lqcd_lattice handle (links, E field, work buffers)."""
+
+    def __init__(self, lattice: LatticeCyclic, precision: int, ctx: Optional[Context]):
+        self.ctx = ctx or Context.default()
+        self.lattice = lattice
+        ext = (C.c_size_t * lattice.dim_st())(*lattice.extents())
+        h = C.c_void_p()
+        _raise(F.lib().lqcd_lattice_create(self.ctx.handle, lattice.dim_st(), ext, lattice.size(),
+                                           precision, C.byref(h)))
+        self.handle = h
+        self.precision = precision
+        self.nlink = lattice.number_of_canonical_links_space()
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                F.lib().lqcd_lattice_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class LatticeStateDefault:
+    """``LatticeStateDefault<D>`` (src/simulation/state.rs:653-850), resident in HBM.
+
+    Implements the ``LatticeState`` / ``LatticeStateNew`` surface:
+    ``link_matrix``, ``set_link_matrix``, ``lattice``, ``beta``, ``CA``,
+    ``hamiltonian_links``, ``monte_carlo_step``, ``average_trace_plaquette``.
+    """
+
+    CA = 3.0
+
+    def __init__(self, lattice: LatticeCyclic, beta: float, link_matrix=None,
+                 precision: int = F64, ctx: Optional[Context] = None):
+        self._dev = _DeviceLattice(lattice, precision, ctx)
+        self._beta = float(beta)
+        if link_matrix is not None:
+            self.set_link_matrix(link_matrix)
+
+    # LatticeStateDefault::new_cold, src/simulation/state.rs:662-671
+    @classmethod
+    def new_cold(cls, size: float, beta: float, number_of_points, d: int = 4,
+                 precision: int = F64, ctx: Optional[Context] = None):
+        return cls(LatticeCyclic(size, number_of_points, d), beta, None, precision, ctx)
+
+    @property
+    def handle(self):
+        return self._dev.handle
+
+    def lattice(self) -> LatticeCyclic:
+        return self._dev.lattice
+
+    def beta(self) -> float:
+        return self._beta
+
+    def precision(self) -> int:
+        return self._dev.precision
+
+    def link_matrix(self) -> np.ndarray:
+        out = np.empty((self._dev.nlink, 18), dtype=np.float64)
+        _raise(F.lib().lqcd_links_download(self.handle, F.ptr(out), self._dev.nlink))
+        return out
+
+    def set_link_matrix(self, links) -> None:
+        a = np.ascontiguousarray(links, dtype=np.float64)
+        if a.ndim != 2 or a.shape[1] != 18:
+            raise StateInitializationError("links must have shape (nsite*D, 18)")
+        _raise(F.lib().lqcd_links_upload(self.handle, F.ptr(a), a.shape[0]))
+
+    def hamiltonian_links(self) -> float:
+        out = C.c_double()
+        _raise(F.lib().lqcd_hamiltonian_links(self.handle, self._beta, C.byref(out)))
+        return out.value
+
+    def average_trace_plaquette(self) -> complex:
+        out = np.zeros(2)
+        _raise(F.lib().lqcd_average_trace_plaquette(self.handle, F.ptr(out)))
+        return complex(out[0], out[1])
+
+    def normalize_link_matrices(self) -> None:
+        _raise(F.lib().lqcd_normalize_link_matrices(self.handle))
+
+    def monte_carlo_step(self, m):
+        """``LatticeState::monte_carlo_step`` (src/simulation/state.rs:103-109)."""
+        return m.next_element(self)
+
+    # Wilson action as a forward / backward pair
+    def wilson_forward(self) -> float:
+        out = C.c_double()
+        _raise(F.lib().lqcd_wilson_forward(self.handle, self._beta, C.byref(out)))
+        return out.value
+
+    def wilson_backward(self) -> np.ndarray:
+        out = np.empty((self._dev.nlink, 8), dtype=np.float64)
+        _raise(F.lib().lqcd_wilson_backward(self.handle, self._beta, F.ptr(out), self._dev.nlink))
+        return out
+
+
+class LatticeStateEFSyncDefault:
+    """``LatticeStateEFSyncDefault<LatticeStateDefault<D>, D>``
+    (src/simulation/state.rs:1048-1449): links plus E field on the device."""
+
+    CA = 3.0
+
+    def __init__(self, lattice_state: LatticeStateDefault, e_field=None, t: int = 0):
+        self._state = lattice_state
+        self._t = t
+        if e_field is not None:
+            self.set_e_field(e_field)
+        else:
+            _raise(F.lib().lqcd_efield_cold(self.handle))
+
+    # LatticeStateEFSyncDefault::new_e_cold, src/simulation/state.rs:1110-1120
+    @classmethod
+    def new_e_cold(cls, lattice_state: LatticeStateDefault):
+        return cls(lattice_state, None)
+
+    @classmethod
+    def new_random_e_state(cls, lattice_state: LatticeStateDefault, seed: int, counter: int = 0,
+                           sigma: Optional[float] = None):
+        """Momenta N(0, sigma) drawn on the device; sigma defaults to the
+        reference's 0.5 / beta (src/simulation/state.rs:1097).  The reference
+        then projects onto the Gauss-law surface; that projection is not
+        applied here."""
+        st = cls(lattice_state, None)
+        st.reset_e_field(seed, counter, sigma)
+        return st
+
+    @property
+    def handle(self):
+        return self._state.handle
+
+    def state_owned(self) -> LatticeStateDefault:
+        return self._state
+
+    def lattice_state(self) -> LatticeStateDefault:
+        return self._state
+
+    def lattice(self) -> LatticeCyclic:
+        return self._state.lattice()
+
+    def beta(self) -> float:
+        return self._state.beta()
+
+    def t(self) -> int:
+        return self._t
+
+    def link_matrix(self) -> np.ndarray:
+        return self._state.link_matrix()
+
+    def set_link_matrix(self, links) -> None:
+        self._state.set_link_matrix(links)
+
+    def e_field(self) -> np.ndarray:
+        n = self.lattice().number_of_canonical_links_space()
+        out = np.empty((n, 8), dtype=np.float64)
+        _raise(F.lib().lqcd_efield_download(self.handle, F.ptr(out), n))
+        return out
+
+    def set_e_field(self, e) -> None:
+        a = np.ascontiguousarray(e, dtype=np.float64)
+        if a.ndim != 2 or a.shape[1] != 8:
+            raise StateInitializationError("e_field must have shape (nsite*D, 8)")
+        _raise(F.lib().lqcd_efield_upload(self.handle, F.ptr(a), a.shape[0]))
+
+    def reset_e_field(self, seed: int, counter: int = 0, sigma: Optional[float] = None) -> None:
+        s = 0.5 / self.beta() if sigma is None else float(sigma)
+        _raise(F.lib().lqcd_efield_random(self.handle, seed, counter, s))
+
+    def hamiltonian_links(self) -> float:
+        return self._state.hamiltonian_links()
+
+    def hamiltonian_efield(self) -> float:
+        out = C.c_double()
+        _raise(F.lib().lqcd_hamiltonian_efield(self.handle, self.beta(), C.byref(out)))
+        return out.value
+
+    def hamiltonian_total(self) -> float:
+        """``LatticeStateWithEField::hamiltonian_total`` (src/simulation/state.rs:229-231)."""
+        return self.hamiltonian_links() + self.hamiltonian_efield()
+
+    def average_trace_plaquette(self) -> complex:
+        return self._state.average_trace_plaquette()
+
+    def derivative_e(self) -> np.ndarray:
+        """``derivative_e`` for every site, layout (nsite*D, 8)."""
+        n = self.lattice().number_of_canonical_links_space()
+        out = np.empty((n, 8), dtype=np.float64)
+        _raise(F.lib().lqcd_derivative_e(self.handle, F.ptr(out), n))
+        return out
+
+    def simulate_symplectic(self, integrator: "SymplecticEulerGpu", delta_t: float):
+        return integrator.integrate_symplectic(self, delta_t)
+
+    def simulate_symplectic_n(self, integrator: "SymplecticEulerGpu", delta_t: float,
+                              numbers_of_times: int):
+        """``SimulationStateSynchronous::simulate_symplectic_n``
+        (src/simulation/state.rs:470-492); advances the state in place."""
+        if numbers_of_times == 0:
+            raise MultiIntegrationError("no integration steps")
+        _raise(F.lib().lqcd_simulate_symplectic_n(self.handle, delta_t, numbers_of_times,
+                                                  integrator.drift, integrator.scheme))
+        self._t += numbers_of_times
+        return self
+
+    def monte_carlo_step(self, m):
+        return m.next_element(self._state)
+
+
+class SymplecticEulerGpu:
+    """``SymplecticEulerRayon`` (src/integrator/symplectic_euler_rayon.rs) on the GPU.
+
+    ``drift``: ``DRIFT_ADDITIVE`` (the reference's U + dt derivative_u) or
+    ``DRIFT_EXP``; ``scheme``: ``SCHEME_REFERENCE`` (kick-drift-kick per step)
+    or ``SCHEME_LEAPFROG`` (half kicks merged between steps).
+    """
+
+    def __init__(self, drift: int = DRIFT_ADDITIVE, scheme: int = SCHEME_REFERENCE):
+        self.drift = drift
+        self.scheme = scheme
+
+    def integrate_symplectic(self, state: LatticeStateEFSyncDefault, delta_t: float):
+        _raise(F.lib().lqcd_simulate_symplectic_n(state.handle, delta_t, 1, self.drift,
+                                                  self.scheme))
+        state._t += 1
+        return state
+
+    def integrate_efield(self, state, delta_t: float):
+        _raise(F.lib().lqcd_integrate_efield(state.handle, delta_t))
+        return state
+
+    def integrate_link(self, state, delta_t: float):
+        _raise(F.lib().lqcd_integrate_link(state.handle, delta_t, self.drift))
+        return state
+
+
+def _splitmix64(x: int):
+    x = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    z = x
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return x, z ^ (z >> 31)
+
+
+class HybridMonteCarloDiagnostic:
+    """``HybridMonteCarloDiagnostic`` (src/simulation/monte_carlo/hybrid_monte_carlo.rs:316-472).
+
+    ``next_element`` draws momenta N(0, 0.5/beta) on the device, integrates
+    ``number_of_steps`` steps of ``delta_t`` and accepts with
+    p = min(1, exp(H_old - H_new)).
+    """
+
+    def __init__(self, delta_t: float, number_of_steps: int, integrator: SymplecticEulerGpu,
+                 seed: int):
+        self._dt = float(delta_t)
+        self._n = int(number_of_steps)
+        self._integrator = integrator
+        self._seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+        self._rng_state = self._seed
+        self._counter = 0
+        self._prob_replace_last = 0.0
+        self._has_replace_last = False
+        self.last_stats: Optional[F.HmcStats] = None
+
+    def delta_t(self) -> float:
+        return self._dt
+
+    def number_of_steps(self) -> int:
+        return self._n
+
+    def integrator(self) -> SymplecticEulerGpu:
+        return self._integrator
+
+    def prob_replace_last(self) -> float:
+        return self._prob_replace_last
+
+    def has_replace_last(self) -> bool:
+        return self._has_replace_last
+
+    def _next_u64(self) -> int:
+        self._rng_state, v = _splitmix64(self._rng_state)
+        return v
+
+    def trajectory(self, state, uniform: int) -> F.HmcStats:
+        """One trajectory from the momenta already in ``state``'s E field."""
+        if self._n == 0:
+            raise MultiIntegrationError("no integration steps")
+        st = F.HmcStats()
+        _raise(F.lib().lqcd_hmc_trajectory(state.handle, state.beta(), self._dt, self._n,
+                                           self._integrator.drift, self._integrator.scheme,
+                                           C.c_uint64(uniform), C.byref(st)))
+        self._prob_replace_last = st.probability
+        self._has_replace_last = bool(st.accepted)
+        self.last_stats = st
+        return st
+
+    def next_element(self, state: LatticeStateDefault) -> LatticeStateDefault:
+        sigma = 0.5 / state.beta()
+        _raise(F.lib().lqcd_efield_random(state.handle, self._seed, self._counter, sigma))
+        self._counter += 1
+        self.trajectory(state, self._next_u64())
+        return state
+
+
+class MetropolisHastingsSweep:
+    """``MetropolisHastingsSweep`` (src/simulation/monte_carlo/metropolis_hastings_sweep.rs),
+    checkerboard order on the device with host-given proposals and uniforms."""
+
+    def __init__(self, number_of_update: int, spread: float):
+        # MetropolisHastingsSweep::new returns None for these (:77-88)
+        if number_of_update == 0 or not (0.0 < spread < 1.0):
+            raise ValueError("number_of_update must be > 0 and spread in (0, 1)")
+        self._n_update = int(number_of_update)
+        self._spread = float(spread)
+        self._prob_replace_mean = 0.0
+        self._number_replace_last = 0
+
+    def number_of_update(self) -> int:
+        return self._n_update
+
+    def spread(self) -> float:
+        return self._spread
+
+    def prob_replace_mean(self) -> float:
+        return self._prob_replace_mean
+
+    def number_replace_last(self) -> int:
+        return self._number_replace_last
+
+    def sweep(self, state: LatticeStateDefault, proposals: np.ndarray,
+              uniforms: np.ndarray) -> LatticeStateDefault:
+        """One sweep with proposals ``(n_update, nlink, 18)`` (orthonormalized,
+        column-major) and one u64 uniform per link."""
+        p = np.ascontiguousarray(proposals, dtype=np.float64)
+        u = np.ascontiguousarray(uniforms, dtype=np.uint64)
+        nlink = state.lattice().number_of_canonical_links_space()
+        if p.shape != (self._n_update, nlink, 18) or u.shape != (nlink,):
+            raise StateInitializationError("proposals / uniforms do not match the lattice")
+        st = F.MhStats()
+        _raise(F.lib().lqcd_metropolis_sweep(state.handle, state.beta(), self._n_update,
+                                             F.ptr(p), F.ptr(u), C.byref(st)))
+        self._prob_replace_mean = st.prob_replace_mean
+        self._number_replace_last = int(st.number_replace_last)
+        return state

@@ -1,0 +1,133 @@
+"""ctypes binding of include/lqcd.h (liblqcd_hip.so, built for gfx950).
+
+There is no fallback: if the library is missing or no GPU is visible, the
+calls raise.  Arrays are numpy float64/uint64 in the reference's layouts.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblqcd_hip.so")
+
+# lqcd_status
+OK = 0
+ERR_ZERO_DIMENSION = 1
+ERR_NON_POSITIVE_SIZE = 2
+ERR_DIM_TOO_SMALL = 3
+ERR_INCOMPATIBLE_SIZE = 4
+ERR_ZERO_INTEGRATION = 5
+ERR_INVALID_ARGUMENT = 6
+ERR_UNSUPPORTED = 7
+ERR_DEVICE = 8
+ERR_OUT_OF_MEMORY = 9
+
+F64, F32 = 0, 1
+DRIFT_ADDITIVE, DRIFT_EXP = 0, 1
+SCHEME_REFERENCE, SCHEME_LEAPFROG = 0, 1
+
+KERNEL_KICK_DRIFT = 0
+KERNEL_KICK = 1
+KERNEL_DRIFT = 2
+KERNEL_PLAQUETTE = 3
+KERNEL_EFIELD_ENERGY = 4
+KERNEL_REUNIT = 5
+KERNEL_METROPOLIS = 6
+KERNEL_MOMENTA = 7
+KERNEL_FORCE = 8
+
+
+class HmcStats(C.Structure):
+    _fields_ = [
+        ("delta_h", C.c_double),
+        ("h_old", C.c_double),
+        ("h_new", C.c_double),
+        ("probability", C.c_double),
+        ("accepted", C.c_int),
+    ]
+
+
+class MhStats(C.Structure):
+    _fields_ = [("prob_replace_mean", C.c_double), ("number_replace_last", C.c_size_t)]
+
+
+_P = C.c_void_p
+_D = C.c_double
+_S = C.c_size_t
+_I = C.c_int
+_U64 = C.c_uint64
+
+# name -> (restype, argtypes); every exported symbol of include/lqcd.h
+SIGNATURES = {
+    "lqcd_last_error": (C.c_char_p, []),
+    "lqcd_version": (C.c_char_p, []),
+    "lqcd_device_count": (_I, [C.POINTER(C.c_int)]),
+    "lqcd_ctx_create": (_I, [_I, C.POINTER(_P)]),
+    "lqcd_ctx_destroy": (_I, [_P]),
+    "lqcd_ctx_synchronize": (_I, [_P]),
+    "lqcd_lattice_create": (_I, [_P, _I, C.POINTER(C.c_size_t), _D, _I, C.POINTER(_P)]),
+    "lqcd_lattice_destroy": (_I, [_P]),
+    "lqcd_lattice_info": (_I, [_P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "lqcd_links_upload": (_I, [_P, _P, _S]),
+    "lqcd_links_download": (_I, [_P, _P, _S]),
+    "lqcd_links_cold": (_I, [_P]),
+    "lqcd_efield_upload": (_I, [_P, _P, _S]),
+    "lqcd_efield_download": (_I, [_P, _P, _S]),
+    "lqcd_efield_cold": (_I, [_P]),
+    "lqcd_efield_random": (_I, [_P, _U64, _U64, _D]),
+    "lqcd_average_trace_plaquette": (_I, [_P, _P]),
+    "lqcd_hamiltonian_links": (_I, [_P, _D, C.POINTER(_D)]),
+    "lqcd_hamiltonian_efield": (_I, [_P, _D, C.POINTER(_D)]),
+    "lqcd_derivative_e": (_I, [_P, _P, _S]),
+    "lqcd_integrate_efield": (_I, [_P, _D]),
+    "lqcd_integrate_link": (_I, [_P, _D, _I]),
+    "lqcd_simulate_symplectic_n": (_I, [_P, _D, _S, _I, _I]),
+    "lqcd_hmc_trajectory": (_I, [_P, _D, _D, _S, _I, _I, _U64, C.POINTER(HmcStats)]),
+    "lqcd_wilson_forward": (_I, [_P, _D, C.POINTER(_D)]),
+    "lqcd_wilson_backward": (_I, [_P, _D, _P, _S]),
+    "lqcd_metropolis_sweep": (_I, [_P, _D, _S, _P, _P, C.POINTER(MhStats)]),
+    "lqcd_normalize_link_matrices": (_I, [_P]),
+    "lqcd_profile_enable": (_I, [_P, C.c_uint]),
+    "lqcd_profile_read": (_I, [_P, _I, C.POINTER(C.c_size_t), C.POINTER(_D)]),
+}
+
+
+class LqcdError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[lqcd status {code}] {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load liblqcd_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with "
+                "`python lattice-qcd-rs_amd/build_lib.py` (hipcc, gfx950)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != OK:
+        msg = lib().lqcd_last_error().decode(errors="replace")
+        raise LqcdError(rc, msg)
+
+
+def ptr(a: np.ndarray):
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    return a.ctypes.data_as(C.c_void_p)

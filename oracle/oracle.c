@@ -40,6 +40,7 @@
 
 #include <float.h>
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -281,10 +282,9 @@ void or_generator(int a, or_mat3* g)
 
 static or_mat3 G_TAB[8];
 static or_mat3 G_SUM; /* GENERATORS.iter().sum(), used by gauss_sum_div */
-static int g_init = 0;
-static void init_tables(void)
+static pthread_once_t g_tables_once = PTHREAD_ONCE_INIT;
+static void fill_tables(void)
 {
-    if (g_init) return;
     or_mat3 s;
     mzero(&s);
     for (int a = 0; a < 8; ++a) {
@@ -292,8 +292,9 @@ static void init_tables(void)
         madd(&s, &G_TAB[a], &s);
     }
     G_SUM = s;
-    g_init = 1;
 }
+/* called from inside OpenMP loops: initialisation must be thread-safe */
+static void init_tables(void) { pthread_once(&g_tables_once, fill_tables); }
 
 /* Su3Adjoint::to_matrix, src/field.rs:106-112: sum over a of G_a * (v_a + 0i),
  * accumulated from the zero matrix in generator order. */

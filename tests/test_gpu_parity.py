@@ -1,0 +1,238 @@
+"""GPU parity: the gfx950 library (through include/lqcd.h) against the CPU oracle
+on identical seeded inputs.
+
+Tolerances (f64 path):
+  plaquette / H          relative 1e-13
+  force (derivative_e)   absolute 1e-12 * max|F|
+  one MD step            max |dU|, |dE| <= 1e-12
+  trajectory dH          |dH_gpu - dH_oracle| <= 1e-9 |H| + 1e-10, same accept decision
+  Metropolis sweep       identical accept count, links max |dU| <= 1e-12
+The f32 storage path is checked against the same f64 oracle with the bounds
+stated in each test.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import lqcd
+from lqcd import _ffi as F
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(msg):
+    print(f"\n[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def make_state(dim, ext, beta=6.0, precision=lqcd.F64, size=1.0, seed=O.SEED_RNG, cold=False):
+    ol = O.OracleLattice(dim, ext, size)
+    u = ol.cold_start() if cold else ol.hot_start(seed)
+    lat = lqcd.LatticeCyclic(size, ext, dim)
+    st = lqcd.LatticeStateDefault(lat, beta, u, precision)
+    return ol, u, st
+
+
+SHAPES = [(4, [4, 4, 4, 4]), (3, [6, 6, 6]), (4, [4, 6, 2, 8]), (2, [8, 6])]
+
+
+@pytest.mark.parametrize("dim,ext", SHAPES)
+def test_links_and_efield_roundtrip(dim, ext):
+    announce(f"roundtrip D={dim} ext={ext}")
+    ol, u, st = make_state(dim, ext)
+    assert np.array_equal(st.link_matrix(), u)
+    e = ol.random_efield(3, 0.7)
+    ef = lqcd.LatticeStateEFSyncDefault(st, e)
+    assert np.array_equal(ef.e_field(), e)
+    _, _, st32 = make_state(dim, ext, precision=lqcd.F32)
+    assert np.array_equal(st32.link_matrix(), u.astype(np.float32).astype(np.float64))
+
+
+@pytest.mark.parametrize("dim,ext", SHAPES + [(4, [8, 8, 8, 8])])
+def test_plaquette_and_hamiltonians(dim, ext):
+    announce(f"plaquette D={dim} ext={ext}")
+    beta = 5.7
+    ol, u, st = make_state(dim, ext, beta)
+    p_ref = ol.average_trace_plaquette(u)
+    p = st.average_trace_plaquette()
+    assert abs(p - p_ref) <= 1e-13 * abs(p_ref)
+    h_ref = ol.hamiltonian_links(u, beta)
+    assert abs(st.hamiltonian_links() - h_ref) <= 1e-13 * abs(h_ref)
+    assert abs(st.wilson_forward() - h_ref) <= 1e-13 * abs(h_ref)
+    e = ol.random_efield(4, 0.5 / beta)
+    ef = lqcd.LatticeStateEFSyncDefault(st, e)
+    he_ref = ol.hamiltonian_efield(e, beta)
+    assert abs(ef.hamiltonian_efield() - he_ref) <= 1e-13 * abs(he_ref)
+
+
+@pytest.mark.parametrize("dim,ext", SHAPES + [(4, [8, 8, 8, 8])])
+def test_derivative_e_and_backward(dim, ext):
+    announce(f"force D={dim} ext={ext}")
+    beta = 5.7
+    ol, u, st = make_state(dim, ext, beta, size=0.5)
+    ref = ol.derivative_e(u)
+    ef = lqcd.LatticeStateEFSyncDefault(st)
+    got = ef.derivative_e()
+    scale = np.max(np.abs(ref))
+    assert np.max(np.abs(got - ref)) <= 1e-12 * scale
+    grad = st.wilson_backward()
+    gref = ol.wilson_backward(u, beta)
+    assert np.max(np.abs(grad - gref)) <= 1e-12 * np.max(np.abs(gref))
+
+
+def test_cold_state_force_zero_and_invariant():
+    announce("cold invariance")
+    ol, u, st = make_state(4, [4, 4, 4, 4], cold=True)
+    ef = lqcd.LatticeStateEFSyncDefault(st)
+    assert np.all(ef.derivative_e() == 0.0)
+    assert st.average_trace_plaquette() == 3.0
+    for drift in (lqcd.DRIFT_ADDITIVE, lqcd.DRIFT_EXP):
+        ef.simulate_symplectic_n(lqcd.SymplecticEulerGpu(drift), 0.1, 3)
+        assert np.array_equal(ef.link_matrix(), u)
+        assert np.all(ef.e_field() == 0.0)
+
+
+@pytest.mark.parametrize("drift", [lqcd.DRIFT_ADDITIVE, lqcd.DRIFT_EXP])
+def test_single_kick_and_drift(drift):
+    announce(f"kick/drift drift={drift}")
+    ol, u, st = make_state(4, [4, 4, 4, 6], size=0.7)
+    e = ol.random_efield(5, 0.4)
+    ef = lqcd.LatticeStateEFSyncDefault(st, e)
+    integ = lqcd.SymplecticEulerGpu(drift)
+    integ.integrate_efield(ef, 0.013)
+    e_ref = ol.integrate_efield(u, e, 0.013)
+    assert np.max(np.abs(ef.e_field() - e_ref)) <= 1e-13
+    integ.integrate_link(ef, 0.021)
+    u_ref = ol.integrate_link(u, e_ref, 0.021, drift)
+    assert np.max(np.abs(ef.link_matrix() - u_ref)) <= 1e-13
+
+
+@pytest.mark.parametrize("drift", [lqcd.DRIFT_ADDITIVE, lqcd.DRIFT_EXP])
+@pytest.mark.parametrize("scheme", [lqcd.SCHEME_REFERENCE, lqcd.SCHEME_LEAPFROG])
+def test_simulate_symplectic_n(drift, scheme):
+    announce(f"symplectic_n drift={drift} scheme={scheme}")
+    ol, u, st = make_state(4, [4, 4, 4, 4], beta=6.0)
+    e = ol.random_efield(6, 0.5 / 6.0)
+    ef = lqcd.LatticeStateEFSyncDefault(st, e)
+    ef.simulate_symplectic_n(lqcd.SymplecticEulerGpu(drift, scheme), 0.01, 5)
+    u_ref, e_ref = ol.simulate_symplectic_n(u, e, 0.01, 5, drift, scheme)
+    assert np.max(np.abs(ef.link_matrix() - u_ref)) <= 1e-12
+    assert np.max(np.abs(ef.e_field() - e_ref)) <= 1e-12
+
+
+def test_zero_integration_error():
+    announce("zero integration")
+    _, _, st = make_state(3, [4, 4, 4])
+    ef = lqcd.LatticeStateEFSyncDefault(st)
+    with pytest.raises(lqcd.MultiIntegrationError):
+        ef.simulate_symplectic_n(lqcd.SymplecticEulerGpu(), 0.01, 0)
+
+
+def test_incompatible_sizes():
+    announce("incompatible sizes")
+    lat = lqcd.LatticeCyclic(1.0, 4, 3)
+    st = lqcd.LatticeStateDefault(lat, 6.0)
+    with pytest.raises(lqcd.StateInitializationError):
+        st.set_link_matrix(np.zeros((10, 18)))
+    with pytest.raises(lqcd.StateInitializationError):
+        lqcd.LatticeStateEFSyncDefault(st, np.zeros((10, 8)))
+
+
+@pytest.mark.parametrize("drift,scheme,dim,ext", [
+    (lqcd.DRIFT_ADDITIVE, lqcd.SCHEME_REFERENCE, 4, [8, 8, 8, 8]),
+    (lqcd.DRIFT_EXP, lqcd.SCHEME_LEAPFROG, 4, [4, 4, 6, 6]),
+    (lqcd.DRIFT_EXP, lqcd.SCHEME_REFERENCE, 3, [6, 6, 6]),
+])
+def test_hmc_trajectory(drift, scheme, dim, ext):
+    announce(f"hmc drift={drift} scheme={scheme} D={dim} ext={ext}")
+    beta = 6.0
+    ol, u, st = make_state(dim, ext, beta)
+    e = ol.random_efield(7, 0.5 / beta)
+    hmc = lqcd.HybridMonteCarloDiagnostic(0.01, 10, lqcd.SymplecticEulerGpu(drift, scheme), 1)
+    for uniform in (0, 2**64 - 1, 2**63):
+        st.set_link_matrix(u)
+        ef = lqcd.LatticeStateEFSyncDefault(st, e)
+        got = hmc.trajectory(st, uniform)
+        u_ref, ref = ol.hmc_trajectory(u, e, beta, 0.01, 10, uniform, drift, scheme)
+        tol = 1e-9 * abs(ref.h_old) + 1e-10
+        assert abs(got.h_old - ref.h_old) <= tol
+        assert abs(got.delta_h - ref.delta_h) <= tol
+        assert got.accepted == ref.accepted
+        assert np.max(np.abs(st.link_matrix() - u_ref)) <= 1e-11
+        assert hmc.has_replace_last() == bool(ref.accepted)
+
+
+@pytest.mark.parametrize("dim,ext,n_update", [(4, [4, 4, 4, 4], 1), (3, [6, 4, 8], 2),
+                                              (4, [8, 8, 8, 8], 1)])
+def test_metropolis_sweep(dim, ext, n_update):
+    announce(f"metropolis D={dim} ext={ext} n_update={n_update}")
+    beta = 6.0
+    ol, u, st = make_state(dim, ext, beta)
+    mh = lqcd.MetropolisHastingsSweep(n_update, 0.1)
+    rr = O.OracleRng(33)
+    for sweep in range(3):
+        props = ol.mh_proposals(40 + sweep, 0.1, n_update)
+        unis = np.array([rr.next_u64() for _ in range(ol.nlink)], dtype=np.uint64)
+        mh.sweep(st, props, unis)
+        u, ref = ol.metropolis_sweep(u, beta, props, unis, O.ORDER_CHECKERBOARD)
+        assert mh.number_replace_last() == ref.number_replace_last
+        assert abs(mh.prob_replace_mean() - ref.prob_replace_mean) <= 1e-12
+        assert np.max(np.abs(st.link_matrix() - u)) <= 1e-12
+
+
+def test_metropolis_rejects_odd_extent():
+    announce("metropolis odd extent")
+    lat = lqcd.LatticeCyclic(1.0, 3, 3)
+    st = lqcd.LatticeStateDefault(lat, 6.0)
+    mh = lqcd.MetropolisHastingsSweep(1, 0.1)
+    with pytest.raises(lqcd.LqcdError):
+        mh.sweep(st, np.zeros((1, 81, 18)), np.zeros(81, dtype=np.uint64))
+
+
+def test_normalize_link_matrices():
+    announce("reunitarize")
+    ol, u, st = make_state(4, [4, 4, 4, 4])
+    noisy = u + 1e-3 * np.random.default_rng(0).normal(size=u.shape)
+    st.set_link_matrix(noisy)
+    st.normalize_link_matrices()
+    ref = ol.normalize_link_matrices(noisy)
+    assert np.max(np.abs(st.link_matrix() - ref)) <= 1e-14
+    # f32 storage: reunitarised in f64, stored f32
+    _, _, st32 = make_state(4, [4, 4, 4, 4], precision=lqcd.F32)
+    st32.set_link_matrix(noisy)
+    st32.normalize_link_matrices()
+    assert np.max(np.abs(st32.link_matrix() - ref)) <= 1e-7
+
+
+def test_device_momenta_match_oracle_stream():
+    announce("device momenta")
+    ol, u, st = make_state(4, [4, 4, 4, 6])
+    ef = lqcd.LatticeStateEFSyncDefault(st)
+    ef.reset_e_field(0xABCDEF, 17, 0.25)
+    ref = ol.device_momenta(0xABCDEF, 17, 0.25)
+    assert np.max(np.abs(ef.e_field() - ref)) <= 1e-14
+
+
+def test_f32_path_errors_vs_f64_oracle():
+    announce("f32 path")
+    beta = 6.0
+    ol, u, st = make_state(4, [8, 8, 8, 8], beta, precision=lqcd.F32)
+    u32 = u.astype(np.float32).astype(np.float64)
+    # plaquette of the stored (f32-rounded) links in f64 arithmetic
+    p_ref = ol.average_trace_plaquette(u32)
+    assert abs(st.average_trace_plaquette() - p_ref) <= 1e-13 * abs(p_ref)
+    ef = lqcd.LatticeStateEFSyncDefault(st)
+    f_ref = ol.derivative_e(u32)
+    assert np.max(np.abs(ef.derivative_e() - f_ref)) <= 2e-6 * np.max(np.abs(f_ref))
+    e = ol.random_efield(8, 0.5 / beta)
+    for drift in (lqcd.DRIFT_ADDITIVE, lqcd.DRIFT_EXP):
+        st.set_link_matrix(u)
+        ef = lqcd.LatticeStateEFSyncDefault(st, e)
+        ef.simulate_symplectic_n(lqcd.SymplecticEulerGpu(drift, lqcd.SCHEME_LEAPFROG), 0.01, 10)
+        u_ref, e_ref = ol.simulate_symplectic_n(u32, e.astype(np.float32).astype(np.float64),
+                                                0.01, 10, drift, O.SCHEME_LEAPFROG)
+        assert np.max(np.abs(ef.link_matrix() - u_ref)) <= 1e-5
+        assert np.max(np.abs(ef.e_field() - e_ref)) <= 1e-5

@@ -48,7 +48,6 @@ def test_integrator_bound_after_mh():
     u = l.cold_start()
     for k in range(10):
         props = l.mh_proposals(100 + k, 0.1)
-        unis = np.array([O.OracleRng(200 + k).next_u64() for _ in range(1)], dtype=np.uint64)
         rr = O.OracleRng(300 + k)
         unis = np.array([rr.next_u64() for _ in range(l.nlink)], dtype=np.uint64)
         u, _ = l.metropolis_sweep(u, beta, props, unis, O.ORDER_LEXICOGRAPHIC)
@@ -197,3 +196,34 @@ def test_device_momenta_statistics():
     assert abs(np.std(e) - 0.5) < 0.01
     e2 = l.device_momenta(1234, 8, 0.5)
     assert not np.allclose(e, e2)
+
+
+def test_parallel_first_call_matches_serial(tmp_path):
+    # regression: the generator tables are initialised on first use, which
+    # happens inside OpenMP loops; a fresh process with many threads must give
+    # the single-thread result
+    import os
+    import subprocess
+    import sys
+    out = tmp_path / "par.npy"
+    code = (
+        "import numpy as np, oracle as O\n"
+        "O.set_threads(32)\n"
+        "l = O.OracleLattice(4, 4)\n"
+        "u = l.hot_start(O.SEED_RNG)\n"
+        "e = l.random_efield(1, 0.5 / 6.0)\n"
+        "u2, e2 = l.simulate_symplectic_n(u, e, 0.01, 2)\n"
+        f"np.save({str(out)!r}, np.concatenate([u2.ravel(), e2.ravel()]))\n"
+    )
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    subprocess.run([sys.executable, "-c", code], cwd=root, check=True)
+    par = np.load(out)
+    O.set_threads(1)
+    try:
+        l = O.OracleLattice(4, 4)
+        u = l.hot_start(O.SEED_RNG)
+        e = l.random_efield(1, 0.5 / 6.0)
+        u2, e2 = l.simulate_symplectic_n(u, e, 0.01, 2)
+    finally:
+        O.set_threads(0)
+    assert np.array_equal(par, np.concatenate([u2.ravel(), e2.ravel()]))
