@@ -92,6 +92,11 @@ int lqcd_links_download(lqcd_lattice* lat, double* links, size_t n_links);
 /* LinkMatrix::new_cold (src/field.rs:716-722) */
 int lqcd_links_cold(lqcd_lattice* lat);
 
+/* LinkMatrix::new_determinist hot start (src/field.rs:646-659 with random_su3,
+ * src/su3.rs:322-347) drawn on the device from a Philox4x32-10 stream keyed
+ * by seed (one independent sub-stream per link). */
+int lqcd_links_random(lqcd_lattice* lat, uint64_t seed);
+
 /* LatticeStateWithEField::set_e_field / e_field (src/simulation/state.rs:190-200) */
 int lqcd_efield_upload(lqcd_lattice* lat, const double* efield, size_t n_links);
 int lqcd_efield_download(lqcd_lattice* lat, double* efield, size_t n_links);
@@ -172,6 +177,21 @@ int lqcd_metropolis_sweep(lqcd_lattice* lat, double beta, size_t n_update,
 
 /* LatticeStateDefault::normalize_link_matrices (src/simulation/state.rs:754-756) */
 int lqcd_normalize_link_matrices(lqcd_lattice* lat);
+
+/* ------------------------------------------------------------- tuning */
+/* Work decomposition of the MD kernels.  No counterpart in the reference
+ * (the rayon pool schedules per-element closures, src/thread.rs:684-699).
+ *   0  one thread per link, grid (site blocks, D)
+ *   1  one thread per link, XCD-grouped 1-D grid (default before tuning)
+ *   2  one thread per site for all D links (shared staple loads), XCD-grouped
+ *   3  as 2, each XCD walking a slab of the slowest direction with that
+ *      direction fastest (neighbour slices share L2)
+ * Variants differ only in rounding (summation order).  The first integration
+ * of a lattice autotunes unless a variant was set or LQCD_MD_VARIANT is set
+ * in the environment. */
+int lqcd_md_autotune(lqcd_lattice* lat, int drift, int* variant);
+int lqcd_md_variant_set(lqcd_lattice* lat, int variant);
+int lqcd_md_variant_get(const lqcd_lattice* lat, int* variant);
 
 /* ------------------------------------------------------------ profiling */
 /* Kernel kinds for lqcd_profile_read (HIP events around each launch). */

@@ -7,6 +7,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -75,10 +76,72 @@ int hamiltonian_terms(lqcd_lattice* L, const void* U, bool with_e)
     return read_results(L, with_e ? 4 : 3);
 }
 
+// Times every MD work decomposition on this lattice (fused kick+drift, kick
+// 0 so E keeps its values; E is restored anyway) and keeps the fastest.
+int autotune_md(lqcd_lattice* L, int drift, int* chosen)
+{
+    int w1, w2;
+    other_buffers(L->cur, &w1, &w2);
+    const size_t ebytes = (size_t)L->g.dim * 8 * (size_t)L->g.npad * L->elem;
+    DevBuf ebak;
+    HIP_TRY(hipMalloc(&ebak.p, ebytes));
+    HIP_TRY(hipMemcpyAsync(ebak.p, L->E, ebytes, hipMemcpyDeviceToDevice, L->ctx->stream));
+    hipEvent_t t0, t1;
+    HIP_TRY(hipEventCreate(&t0));
+    HIP_TRY(hipEventCreate(&t1));
+    int best = VAR_LINK_XCD;
+    float best_ms = 1e30f;
+    int rc = LQCD_OK;
+    for (int v = 0; v < VAR_COUNT && rc == LQCD_OK; ++v) {
+        const int reps = 3;
+        hipError_t e = launch_kick_drift_variant(L, v, L->U[L->cur], L->U[w1], L->E, 0.0, 1e-3, drift);
+        if (e == hipSuccess) e = hipEventRecord(t0, L->ctx->stream);
+        for (int r = 0; r < reps && e == hipSuccess; ++r)
+            e = launch_kick_drift_variant(L, v, L->U[L->cur], L->U[w1], L->E, 0.0, 1e-3, drift);
+        if (e == hipSuccess) e = hipEventRecord(t1, L->ctx->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(t1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+        if (e != hipSuccess) {
+            rc = fail(LQCD_ERR_DEVICE, std::string("autotune: ") + hipGetErrorString(e));
+            break;
+        }
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = v;
+        }
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (rc != LQCD_OK) return rc;
+    HIP_TRY(hipMemcpyAsync(L->E, ebak.p, ebytes, hipMemcpyDeviceToDevice, L->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    L->md_variant = best;
+    if (chosen) *chosen = best;
+    return LQCD_OK;
+}
+
+// The MD work decomposition: LQCD_MD_VARIANT pins it, otherwise the first
+// integration of a lattice autotunes it.
+int ensure_md_variant(lqcd_lattice* L, int drift)
+{
+    if (L->md_variant >= 0) return LQCD_OK;
+    const char* env = std::getenv("LQCD_MD_VARIANT");
+    if (env && *env) {
+        const int v = std::atoi(env);
+        if (v >= 0 && v < VAR_COUNT) {
+            L->md_variant = v;
+            return LQCD_OK;
+        }
+    }
+    return autotune_md(L, drift, nullptr);
+}
+
 // n steps of the chosen scheme starting from U[cur]; *last gets the buffer
 // holding the final links.  U[cur] is left untouched.
 int integrate_steps(lqcd_lattice* L, double dt, size_t n, int drift, int scheme, int* last)
 {
+    if (int rc = ensure_md_variant(L, drift)) return rc;
     int w1, w2;
     other_buffers(L->cur, &w1, &w2);
     int src = L->cur, dst = w1;
@@ -327,6 +390,15 @@ int lqcd_links_cold(lqcd_lattice* L)
     return LQCD_OK;
 }
 
+int lqcd_links_random(lqcd_lattice* L, uint64_t seed)
+{
+    CHECK_ARG(L, "lattice is null");
+    DeviceGuard dg(L->ctx->device);
+    HIP_TRY(launch_links_random(L, L->U[L->cur], seed));
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    return LQCD_OK;
+}
+
 int lqcd_efield_upload(lqcd_lattice* L, const double* efield, size_t n_links)
 {
     CHECK_ARG(L && efield, "null argument");
@@ -430,6 +502,7 @@ int lqcd_integrate_efield(lqcd_lattice* L, double dt)
     CHECK_ARG(L, "lattice is null");
     CHECK_ARG(std::isfinite(dt), "dt must be finite");
     DeviceGuard dg(L->ctx->device);
+    if (int rc = ensure_md_variant(L, LQCD_DRIFT_EXP)) return rc;
     HIP_TRY(launch_kick(L, L->U[L->cur], L->E, dt));
     HIP_TRY(hipStreamSynchronize(L->ctx->stream));
     return LQCD_OK;
@@ -550,6 +623,29 @@ int lqcd_normalize_link_matrices(lqcd_lattice* L)
     DeviceGuard dg(L->ctx->device);
     HIP_TRY(launch_reunitarize(L, L->U[L->cur]));
     HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    return LQCD_OK;
+}
+
+int lqcd_md_autotune(lqcd_lattice* L, int drift, int* variant)
+{
+    CHECK_ARG(L, "lattice is null");
+    CHECK_ARG(drift == LQCD_DRIFT_ADDITIVE || drift == LQCD_DRIFT_EXP, "unknown drift mode");
+    DeviceGuard dg(L->ctx->device);
+    return autotune_md(L, drift, variant);
+}
+
+int lqcd_md_variant_set(lqcd_lattice* L, int variant)
+{
+    CHECK_ARG(L, "lattice is null");
+    CHECK_ARG(variant >= -1 && variant < VAR_COUNT, "unknown MD variant");
+    L->md_variant = variant;
+    return LQCD_OK;
+}
+
+int lqcd_md_variant_get(const lqcd_lattice* L, int* variant)
+{
+    CHECK_ARG(L && variant, "null argument");
+    *variant = L->md_variant;
     return LQCD_OK;
 }
 

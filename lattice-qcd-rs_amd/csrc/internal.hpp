@@ -41,6 +41,7 @@ struct lqcd_lattice {
     double* mh_p = nullptr;        // per-link Metropolis probabilities
     uint32_t* mh_acc = nullptr;    // per-link Metropolis accept flags
     int nblocks_reduce = 0;
+    int md_variant = -1;           // work decomposition of the MD kernels (-1: not tuned)
 };
 
 namespace lqcd {
@@ -72,6 +73,7 @@ hipError_t launch_efield_zero(const lqcd_lattice* L, void* E);
 hipError_t launch_device_momenta(const lqcd_lattice* L, void* E, uint64_t seed, uint64_t counter,
                                  double sigma);
 hipError_t launch_reunitarize(const lqcd_lattice* L, void* U);
+hipError_t launch_links_random(const lqcd_lattice* L, void* U, uint64_t seed);
 
 // ---- observables (k_observables.hip) ----
 // out[0..1] = sum Tr P (complex), out[2] = sum (1 - Re Tr P / 3)
@@ -83,6 +85,10 @@ hipError_t launch_mh_stats(const lqcd_lattice* L, double* d_out2);
 
 // ---- molecular dynamics (k_md.hip) ----
 enum DriftMode { DRIFT_ADDITIVE = 0, DRIFT_EXP = 1 };
+enum MdVariant { VAR_LINK = 0, VAR_LINK_XCD = 1, VAR_SITE_XCD = 2, VAR_SITE_SLAB = 3, VAR_COUNT = 4 };
+hipError_t launch_kick_variant(const lqcd_lattice* L, int variant, const void* U, void* E, double kick);
+hipError_t launch_kick_drift_variant(const lqcd_lattice* L, int variant, const void* U_in, void* U_out,
+                                     void* E, double kick, double dt, int drift_mode);
 // out[(s*D+mu)*8+a] = scale * derivative_e^a
 hipError_t launch_force_store(const lqcd_lattice* L, const void* U, double* d_out, double scale);
 // E += kick * derivative_e(U)

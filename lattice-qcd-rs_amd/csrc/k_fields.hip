@@ -136,6 +136,53 @@ __global__ __launch_bounds__(kBlock) void k_device_momenta(S* __restrict__ E, Ge
     store_e<double, S>(E, g.npad, mu, s, e);
 }
 
+// Hot start on the device: random_su3 (src/su3.rs:322-347) -- two complex
+// 3-vectors with re, im ~ U[-1, 1), redrawn while |v1| <= eps or
+// |v1 . v2| <= eps (non-conjugated dot), then the column Gram-Schmidt of
+// orthonormalize_matrix.  The 12 uniforms of attempt r come from the Philox
+// blocks j = 16 + 6 r .. 21 + 6 r of link l (counter as for the momenta,
+// counter word 2..3 = 0); at most 4 attempts.
+template <int D, typename S>
+__global__ __launch_bounds__(kBlock) void k_links_random(vec2_t<S>* __restrict__ U, Geom g, uint64_t seed)
+{
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int mu = blockIdx.y;
+    if (s >= g.nsite) return;
+    const uint64_t l = (uint64_t)s * D + mu;
+    const uint2 key = make_uint2((uint32_t)seed, (uint32_t)(seed >> 32));
+    const double eps = 2.220446049250313e-16;
+    mat3<double> m;
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        double v[12];
+#pragma unroll
+        for (uint32_t j = 0; j < 6; ++j) {
+            const uint32_t blk = 16u + 6u * (uint32_t)attempt + j;
+            const uint4 ctr = make_uint4((uint32_t)l, (uint32_t)((l >> 32) & 0xFFFFu) | (blk << 16), 0u, 0u);
+            const uint4 o = philox4x32(ctr, key);
+            v[2 * j] = 2.0 * u53(o.x, o.y) - 1.0;
+            v[2 * j + 1] = 2.0 * u53(o.z, o.w) - 1.0;
+        }
+        // v1 = (v0 + i v1, v2 + i v3, v4 + i v5), v2 = (v6 + i v7, ...)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            m.e[r * 3 + 0] = {v[2 * r], v[2 * r + 1]};
+            m.e[r * 3 + 1] = {v[6 + 2 * r], v[6 + 2 * r + 1]};
+            m.e[r * 3 + 2] = {0.0, 0.0};
+        }
+        double n2 = 0.0, dre = 0.0, dim_ = 0.0;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const cplx<double> a = m.e[r * 3 + 0], b = m.e[r * 3 + 1];
+            n2 += a.re * a.re + a.im * a.im;
+            dre += a.re * b.re - a.im * b.im;
+            dim_ += a.re * b.im + a.im * b.re;
+        }
+        if (sqrt(n2) > eps && sqrt(dre * dre + dim_ * dim_) > eps) break;
+    }
+    orthonormalize(m);
+    store_link<double, S>(U, g.npad, mu, s, m);
+}
+
 // LinkMatrix::normalize (src/field.rs:897-901) in double arithmetic
 template <typename S>
 __global__ __launch_bounds__(kBlock) void k_reunitarize(vec2_t<S>* __restrict__ U, Geom g)
@@ -202,6 +249,13 @@ hipError_t launch_device_momenta(const lqcd_lattice* L, void* E, uint64_t seed, 
     LQCD_DISPATCH_DS(L, k_device_momenta<D, S><<<link_grid(L), kBlock, 0, L->ctx->stream>>>(
                             (S*)E, L->g, seed, counter, sigma));
     prof_end(L->ctx, &ev);
+    return hipGetLastError();
+}
+
+hipError_t launch_links_random(const lqcd_lattice* L, void* U, uint64_t seed)
+{
+    LQCD_DISPATCH_DS(L, k_links_random<D, S><<<link_grid(L), kBlock, 0, L->ctx->stream>>>(
+                            (vec2_t<S>*)U, L->g, seed));
     return hipGetLastError();
 }
 

@@ -48,7 +48,13 @@ __device__ __forceinline__ cplx<T> elem(const mat3<T>& a, int i, int k)
     return a.e[i * 3 + k];
 }
 
-// c = op(a) * op(b); c may not alias a or b
+// c = op(a) * op(b); c may not alias a or b.
+// Accumulation order: the terms k != i first, then k == i with its real
+// product last.  For near-identity factors the diagonal entry is then one
+// rounding of 1 - e1 - e2 + e1 e2 + (small terms); rounding the leading
+// product on its own would drop the positive e1 e2 every time (the rest is
+// exactly representable), a systematic shrinking of f32 links under repeated
+// link updates.
 template <bool DA, bool DB, typename T>
 __device__ __forceinline__ void mul(const mat3<T>& a, const mat3<T>& b, mat3<T>& c)
 {
@@ -58,7 +64,8 @@ __device__ __forceinline__ void mul(const mat3<T>& a, const mat3<T>& b, mat3<T>&
         for (int j = 0; j < 3; ++j) {
             T re = T(0), im = T(0);
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
+            for (int kk = 1; kk < 3; ++kk) {
+                const int k = (i + kk) % 3;
                 const cplx<T> x = elem<DA>(a, i, k);
                 const cplx<T> y = elem<DB>(b, k, j);
                 re = fma(x.re, y.re, re);
@@ -66,6 +73,12 @@ __device__ __forceinline__ void mul(const mat3<T>& a, const mat3<T>& b, mat3<T>&
                 im = fma(x.re, y.im, im);
                 im = fma(x.im, y.re, im);
             }
+            const cplx<T> x = elem<DA>(a, i, i);
+            const cplx<T> y = elem<DB>(b, i, j);
+            re = fma(-x.im, y.im, re);
+            im = fma(x.re, y.im, im);
+            im = fma(x.im, y.re, im);
+            re = fma(x.re, y.re, re);
             c.e[i * 3 + j] = {re, im};
         }
 }
@@ -193,26 +206,18 @@ __device__ __constant__ const double kInvFactorial[26] = {
     6.446950284384474e-26,
 };
 
-// exp(i X) for hermitian traceless X given by its adjoint coordinates v
-// (X = sum v_a T^a) by the Cayley-Hamilton recursion over 1/k! of
-// su3_exp_i, src/su3.rs:832-855: with d = i det X and t = -Tr(X^2)/2,
-// q0 <- 1/k! + d q2, q1 <- i (q0 - t q2), q2 <- i q1 for k = 24 .. 0,
-// exp(iX) = q0 + q1 X + q2 X^2.
-template <typename T>
-__device__ __forceinline__ void exp_i(const T* v, mat3<T>& out)
+// Cayley-Hamilton recursion of su3_exp_i (src/su3.rs:832-855) started at
+// order K: q0 = 1/K!, then q0 <- 1/k! + d q2, q1 <- i (q0 - t q2),
+// q2 <- i q1 for k = K-1 .. 0 (the Taylor series to order K reduced by
+// X^3 = -t X - i d).  Fully unrolled.
+template <int K, typename T>
+__device__ __forceinline__ void ch_recursion(cplx<T> d, T t, cplx<T>& q0, cplx<T>& q1, cplx<T>& q2)
 {
-    mat3<T> X, X2;
-    adjoint_to_matrix(v, X);
-    mul<false, false>(X, X, X2);
-    T ss = T(0);
+    q0 = {T(kInvFactorial[K]), T(0)};
+    q1 = {T(0), T(0)};
+    q2 = {T(0), T(0)};
 #pragma unroll
-    for (int a = 0; a < 8; ++a) ss = fma(v[a], v[a], ss);
-    const T t = T(-0.25) * ss;
-    const cplx<T> dx = det(X);
-    const cplx<T> d = {-dx.im, dx.re};
-    cplx<T> q0 = {T(kInvFactorial[25]), T(0)}, q1 = {T(0), T(0)}, q2 = {T(0), T(0)};
-#pragma unroll
-    for (int k = 24; k >= 0; --k) {
+    for (int k = K - 1; k >= 0; --k) {
         const cplx<T> dq2 = cmul(d, q2);
         const cplx<T> q0n = {T(kInvFactorial[k]) + dq2.re, dq2.im};
         const cplx<T> w = {fma(-t, q2.re, q0.re), fma(-t, q2.im, q0.im)};
@@ -222,6 +227,44 @@ __device__ __forceinline__ void exp_i(const T* v, mat3<T>& out)
         q1 = q1n;
         q2 = q2n;
     }
+}
+
+// exp(i X) for hermitian traceless X given by its adjoint coordinates v
+// (X = sum v_a T^a): with d = i det X and t = -Tr(X^2)/2,
+// exp(iX) = q0 + q1 X + q2 X^2 (the recursion of su3_exp_i,
+// src/su3.rs:832-855).  The reference always sums to order 25.  Here the
+// series stops at K = 12 (f64) / 7 (f32), which is within the working
+// precision while r = |X|_F (a bound of the spectral radius) is at most
+// 0.25 / 0.35: the remainder r^(K+1)/(K+1)! e^r is below 1e-17 / 1e-8.
+// Larger X are scaled by 2^-m into that range and the result squared m
+// times (m = 0 for molecular-dynamics step sizes).
+template <typename T>
+__device__ __forceinline__ void exp_i(const T* v_in, mat3<T>& out)
+{
+    constexpr bool f32 = sizeof(T) == 4;
+    constexpr int K = f32 ? 7 : 12;
+    const T max_norm2 = f32 ? T(0.35 * 0.35) : T(0.25 * 0.25);
+    T ss = T(0);
+#pragma unroll
+    for (int a = 0; a < 8; ++a) ss = fma(v_in[a], v_in[a], ss);
+    int m = 0;
+    T scale = T(1);
+    while (T(0.5) * ss * scale * scale > max_norm2 && m < 40) {
+        scale *= T(0.5);
+        ++m;
+    }
+    T v[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) v[a] = scale * v_in[a];
+    ss *= scale * scale;
+    mat3<T> X, X2;
+    adjoint_to_matrix(v, X);
+    mul<false, false>(X, X, X2);
+    const T t = T(-0.25) * ss;
+    const cplx<T> dx = det(X);
+    const cplx<T> d = {-dx.im, dx.re};
+    cplx<T> q0, q1, q2;
+    ch_recursion<K>(d, t, q0, q1, q2);
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         const cplx<T> x = X.e[k], y = X2.e[k];
@@ -236,6 +279,11 @@ __device__ __forceinline__ void exp_i(const T* v, mat3<T>& out)
     out.e[0].re += q0.re; out.e[0].im += q0.im;
     out.e[4].re += q0.re; out.e[4].im += q0.im;
     out.e[8].re += q0.re; out.e[8].im += q0.im;
+    for (int i = 0; i < m; ++i) {
+        mat3<T> sq;
+        mul<false, false>(out, out, sq);
+        out = sq;
+    }
 }
 
 // column Gram-Schmidt of orthonormalize_matrix, src/su3.rs:279-303:

@@ -188,6 +188,15 @@ class LatticeStateDefault:
                  precision: int = F64, ctx: Optional[Context] = None):
         return cls(LatticeCyclic(size, number_of_points, d), beta, None, precision, ctx)
 
+    # LatticeStateDefault::new_determinist, src/simulation/state.rs:706-715,
+    # with the random SU(3) links drawn on the device from ``seed``
+    @classmethod
+    def new_determinist(cls, size: float, beta: float, number_of_points, seed: int, d: int = 4,
+                        precision: int = F64, ctx: Optional[Context] = None):
+        st = cls(LatticeCyclic(size, number_of_points, d), beta, None, precision, ctx)
+        _raise(F.lib().lqcd_links_random(st.handle, C.c_uint64(seed)))
+        return st
+
     @property
     def handle(self):
         return self._dev.handle
@@ -224,6 +233,21 @@ class LatticeStateDefault:
 
     def normalize_link_matrices(self) -> None:
         _raise(F.lib().lqcd_normalize_link_matrices(self.handle))
+
+    # MD work decomposition (0 per link, 1 per link XCD-grouped, 2 per site,
+    # 3 per site with slab order)
+    def md_variant(self) -> int:
+        v = C.c_int()
+        _raise(F.lib().lqcd_md_variant_get(self.handle, C.byref(v)))
+        return v.value
+
+    def set_md_variant(self, variant: int) -> None:
+        _raise(F.lib().lqcd_md_variant_set(self.handle, variant))
+
+    def md_autotune(self, drift: int = DRIFT_EXP) -> int:
+        v = C.c_int()
+        _raise(F.lib().lqcd_md_autotune(self.handle, drift, C.byref(v)))
+        return v.value
 
     def monte_carlo_step(self, m):
         """``LatticeState::monte_carlo_step`` (src/simulation/state.rs:103-109)."""

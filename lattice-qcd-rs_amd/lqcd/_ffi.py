@@ -74,6 +74,7 @@ SIGNATURES = {
     "lqcd_links_upload": (_I, [_P, _P, _S]),
     "lqcd_links_download": (_I, [_P, _P, _S]),
     "lqcd_links_cold": (_I, [_P]),
+    "lqcd_links_random": (_I, [_P, _U64]),
     "lqcd_efield_upload": (_I, [_P, _P, _S]),
     "lqcd_efield_download": (_I, [_P, _P, _S]),
     "lqcd_efield_cold": (_I, [_P]),
@@ -90,6 +91,9 @@ SIGNATURES = {
     "lqcd_wilson_backward": (_I, [_P, _D, _P, _S]),
     "lqcd_metropolis_sweep": (_I, [_P, _D, _S, _P, _P, C.POINTER(MhStats)]),
     "lqcd_normalize_link_matrices": (_I, [_P]),
+    "lqcd_md_autotune": (_I, [_P, _I, C.POINTER(C.c_int)]),
+    "lqcd_md_variant_set": (_I, [_P, _I]),
+    "lqcd_md_variant_get": (_I, [_P, C.POINTER(C.c_int)]),
     "lqcd_profile_enable": (_I, [_P, C.c_uint]),
     "lqcd_profile_read": (_I, [_P, _I, C.POINTER(C.c_size_t), C.POINTER(_D)]),
 }

@@ -118,6 +118,7 @@ def _declare(L):
         "or_mh_proposals": (None, [LP, _U64, _D, _S, _P]),
         "or_philox4x32": (None, [_P, _P, _P]),
         "or_device_momenta": (None, [LP, _U64, _U64, _D, _P]),
+        "or_device_hot_start": (None, [LP, _U64, _P]),
         "or_bernoulli_accept": (_I, [_D, _U64]),
         "or_clamp_probability": (_D, [_D]),
         "or_link_matrix": (None, [LP, _P, _S, _I, _I, _P]),
@@ -234,6 +235,11 @@ class OracleLattice:
         e = np.zeros((self.nlink, 8))
         lib().or_random_efield(self.ref, seed, sigma, _ptr(e))
         return e
+
+    def device_hot_start(self, seed: int) -> np.ndarray:
+        u = np.zeros((self.nlink, 18))
+        lib().or_device_hot_start(self.ref, seed, _ptr(u))
+        return u
 
     def device_momenta(self, seed: int, counter: int, sigma: float) -> np.ndarray:
         e = np.zeros((self.nlink, 8))

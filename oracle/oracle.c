@@ -719,6 +719,40 @@ void or_device_momenta(const or_lattice* l, uint64_t seed, uint64_t counter, dou
         }
 }
 
+/* Device hot start stream: link l, attempt r (0..3), block j (0..5) ->
+ * Philox counter {lo32(l), hi16(l) | (16 + 6r + j) << 16, 0, 0}, key = seed;
+ * each block gives two uniforms 2*u53 - 1 in [-1, 1).  v1 = (x0 + i x1,
+ * x2 + i x3, x4 + i x5), v2 = (x6 + i x7, ...); accepted as random_su3
+ * (src/su3.rs:334-347) does, then ortho_matrix_from_2_vector. */
+void or_device_hot_start(const or_lattice* l, uint64_t seed, double* links)
+{
+    size_t nl = l->nsite * (size_t)l->dim;
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    for (size_t li = 0; li < nl; ++li) {
+        vec3 v1, v2;
+        for (uint32_t attempt = 0; attempt < 4; ++attempt) {
+            double x[12];
+            for (uint32_t j = 0; j < 6; ++j) {
+                uint32_t blk = 16u + 6u * attempt + j;
+                uint32_t ctr[4] = {(uint32_t)li, (uint32_t)((li >> 32) & 0xFFFFu) | (blk << 16), 0u, 0u};
+                uint32_t o[4];
+                or_philox4x32(ctr, key, o);
+                x[2 * j] = 2.0 * u53(o[0], o[1]) - 1.0;
+                x[2 * j + 1] = 2.0 * u53(o[2], o[3]) - 1.0;
+            }
+            for (int r = 0; r < 3; ++r) {
+                v1.v[r] = cx(x[2 * r], x[2 * r + 1]);
+                v2.v[r] = cx(x[6 + 2 * r], x[6 + 2 * r + 1]);
+            }
+            or_cplx d = vdot(&v1, &v2, 0);
+            if (vnorm(&v1) > F64_EPS && hypot(d.re, d.im) > F64_EPS) break;
+        }
+        or_mat3 m;
+        ortho_matrix_from_2_vector(&v1, &v2, &m);
+        store_mat(links + 18 * li, &m);
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* stencils: src/field.rs:726-804, src/simulation/state.rs                  */
 /* ------------------------------------------------------------------------ */

@@ -108,19 +108,44 @@ def test_single_kick_and_drift(drift):
     integ.integrate_link(ef, 0.021)
     u_ref = ol.integrate_link(u, e_ref, 0.021, drift)
     assert np.max(np.abs(ef.link_matrix() - u_ref)) <= 1e-13
+    if drift == lqcd.DRIFT_EXP:
+        # |X| ~ 1.5: the device series is scaled by 2^-m and squared m times;
+        # the reference's 25-term series is still exact at this size
+        big = ol.random_efield(9, 10.0)
+        st.set_link_matrix(u)
+        ef.set_e_field(big)
+        integ.integrate_link(ef, 0.021)
+        u_ref = ol.integrate_link(u, big, 0.021, drift)
+        assert np.max(np.abs(ef.link_matrix() - u_ref)) <= 1e-13
 
 
 @pytest.mark.parametrize("drift", [lqcd.DRIFT_ADDITIVE, lqcd.DRIFT_EXP])
 @pytest.mark.parametrize("scheme", [lqcd.SCHEME_REFERENCE, lqcd.SCHEME_LEAPFROG])
-def test_simulate_symplectic_n(drift, scheme):
-    announce(f"symplectic_n drift={drift} scheme={scheme}")
-    ol, u, st = make_state(4, [4, 4, 4, 4], beta=6.0)
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+def test_simulate_symplectic_n(drift, scheme, variant):
+    announce(f"symplectic_n drift={drift} scheme={scheme} variant={variant}")
+    for dim, ext in [(4, [4, 4, 4, 6]), (3, [6, 4, 6]), (2, [8, 6]), (4, [8, 4, 8, 8])]:
+        ol, u, st = make_state(dim, ext, beta=6.0)
+        st.set_md_variant(variant)
+        e = ol.random_efield(6, 0.5 / 6.0)
+        ef = lqcd.LatticeStateEFSyncDefault(st, e)
+        ef.simulate_symplectic_n(lqcd.SymplecticEulerGpu(drift, scheme), 0.01, 5)
+        u_ref, e_ref = ol.simulate_symplectic_n(u, e, 0.01, 5, drift, scheme)
+        assert np.max(np.abs(ef.link_matrix() - u_ref)) <= 1e-12
+        assert np.max(np.abs(ef.e_field() - e_ref)) <= 1e-12
+        assert st.md_variant() == variant
+
+
+def test_md_autotune_picks_a_variant():
+    announce("md autotune")
+    ol, u, st = make_state(4, [8, 8, 8, 8])
     e = ol.random_efield(6, 0.5 / 6.0)
     ef = lqcd.LatticeStateEFSyncDefault(st, e)
-    ef.simulate_symplectic_n(lqcd.SymplecticEulerGpu(drift, scheme), 0.01, 5)
-    u_ref, e_ref = ol.simulate_symplectic_n(u, e, 0.01, 5, drift, scheme)
-    assert np.max(np.abs(ef.link_matrix() - u_ref)) <= 1e-12
-    assert np.max(np.abs(ef.e_field() - e_ref)) <= 1e-12
+    v = st.md_autotune(lqcd.DRIFT_EXP)
+    assert v in (0, 1, 2, 3) and st.md_variant() == v
+    # the tuner leaves the state untouched
+    assert np.array_equal(ef.e_field(), e)
+    assert np.array_equal(st.link_matrix(), u)
 
 
 def test_zero_integration_error():
@@ -205,6 +230,15 @@ def test_normalize_link_matrices():
     st32.set_link_matrix(noisy)
     st32.normalize_link_matrices()
     assert np.max(np.abs(st32.link_matrix() - ref)) <= 1e-7
+
+
+def test_device_hot_start_match_oracle_stream():
+    announce("device hot start")
+    for dim, ext in [(4, [4, 4, 4, 6]), (3, [6, 6, 6])]:
+        ol = O.OracleLattice(dim, ext)
+        st = lqcd.LatticeStateDefault.new_determinist(1.0, 6.0, ext, 0x1234, dim)
+        ref = ol.device_hot_start(0x1234)
+        assert np.max(np.abs(st.link_matrix() - ref)) <= 1e-14
 
 
 def test_device_momenta_match_oracle_stream():
