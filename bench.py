@@ -51,7 +51,17 @@ def parse():
     p.add_argument("--cpu-md-steps", type=int, default=3,
                    help="MD steps of the CPU baseline sample (0 = skip)")
     p.add_argument("--seed", type=int, default=0x457893F44AB067F0)
+    p.add_argument("--copy-only", action="store_true",
+                   help="only run the 1 GiB streaming copy (byte-counter calibration)")
     return p.parse_args()
+
+
+def stream_copy_gbs(lqcd, nbytes=1 << 30, iters=3):
+    import ctypes
+    out = ctypes.c_double()
+    ctx = lqcd.Context.default()
+    lqcd._raise(lqcd._ffi.lib().lqcd_stream_copy(ctx.handle, nbytes, iters, ctypes.byref(out)))
+    return out.value
 
 
 def run_leg(lqcd, precision, args, steps, warmup, profile):
@@ -89,6 +99,7 @@ def run_leg(lqcd, precision, args, steps, warmup, profile):
     plaq = st.average_trace_plaquette().real / 3.0
     nlink = st.lattice().number_of_canonical_links_space()
     return {
+        "md_variant": st.md_variant(),
         "seconds": t1 - t0,
         "traj_per_s": steps / (t1 - t0),
         "mean_abs_dh": float(sum(abs(x) for x in dh) / len(dh)),
@@ -98,6 +109,19 @@ def run_leg(lqcd, precision, args, steps, warmup, profile):
         "kernels": kern,
         "dh": dh,
     }
+
+
+def measured_traffic(variant):
+    """HBM bytes per kick+drift launch (fetched + written) from the committed
+    rocprofv3 counter summary, when it covers the selected MD variant."""
+    path = os.path.join(ROOT, "profiles", "kick_drift_32x4_f32_pmc.md")
+    try:
+        for line in open(path):
+            if line.startswith(f"traffic_bytes_per_launch variant={variant}:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(args):
@@ -130,6 +154,10 @@ def main():
         sys.exit(2)
     import lqcd
     F = lqcd._ffi
+    if args.copy_only:
+        print(json.dumps({"stream_copy_GB_s": round(stream_copy_gbs(lqcd), 1),
+                          "bytes": 1 << 30, "iters": 3}))
+        return
 
     mixed = run_leg(lqcd, lqcd.F32, args, args.steps, args.warmup, profile=True)
     f64 = run_leg(lqcd, lqcd.F64, args, args.f64_steps, args.warmup, profile=True)
@@ -149,14 +177,17 @@ def main():
                 "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
     kd = kernel_line(mixed, F.KERNEL_KICK_DRIFT, REALS_KICK_DRIFT, 4)
+    variant = mixed["md_variant"]
+    traffic = measured_traffic(variant) if args.extent == 32 else None
     roofline = {
         "bound": "hbm",
         "achieved": kd["GB_s"],
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": kd["hbm_frac"],
-        "traffic": None,
-        "kernel": "k_kick_drift<4,float,float,EXP> (fused force + momentum kick + exp link drift)",
+        "traffic": traffic,
+        "kernel": (f"k_md_{'site' if variant >= 2 else 'link'}_xcd<4,float,float,EXP> "
+                   f"(fused force + momentum kick + exp link drift, MD variant {variant})"),
         "bytes_per_launch": nlink * REALS_KICK_DRIFT * 4,
     }
     kernels = {}
@@ -170,6 +201,7 @@ def main():
         kernels["f64_" + name] = kernel_line(f64, kind, reals, 8)
 
     cpu = cpu_baseline(args) if args.cpu_md_steps > 0 else None
+    copy_gbs = stream_copy_gbs(lqcd)
 
     line = {
         "metric": METRIC,
@@ -203,6 +235,7 @@ def main():
         "dH_f32_minus_f64_first_trajectories": [
             a - b for a, b in zip(mixed["dh"][: len(f64["dh"])], f64["dh"])][:3],
         "roofline": roofline,
+        "stream_copy_GB_s": round(copy_gbs, 1),
         "kernels": kernels,
         "cpu_baseline": cpu,
     }

@@ -207,6 +207,12 @@ typedef enum {
     LQCD_KERNEL_FORCE = 8
 } lqcd_kernel_kind;
 
+/* Copies `bytes` (a multiple of 16) between two device buffers `iters` times
+ * with a streaming kernel and returns the read+write rate in GB/s: the
+ * achievable HBM bandwidth of the device, and a reference of known size for
+ * byte counters.  No counterpart in the reference. */
+int lqcd_stream_copy(lqcd_ctx* ctx, size_t bytes, int iters, double* gb_per_s);
+
 /* Record HIP events around every launch of the kinds in `mask` (bit = kind)
  * on the context's stream; mask 0 stops recording.  Clears past records. */
 int lqcd_profile_enable(lqcd_ctx* ctx, unsigned mask);

@@ -74,6 +74,7 @@ hipError_t launch_device_momenta(const lqcd_lattice* L, void* E, uint64_t seed, 
                                  double sigma);
 hipError_t launch_reunitarize(const lqcd_lattice* L, void* U);
 hipError_t launch_links_random(const lqcd_lattice* L, void* U, uint64_t seed);
+hipError_t launch_stream_copy(hipStream_t stream, const void* src, void* dst, size_t bytes);
 
 // ---- observables (k_observables.hip) ----
 // out[0..1] = sum Tr P (complex), out[2] = sum (1 - Re Tr P / 3)

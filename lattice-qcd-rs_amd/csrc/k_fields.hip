@@ -195,6 +195,22 @@ __global__ __launch_bounds__(kBlock) void k_reunitarize(vec2_t<S>* __restrict__ 
     store_link<double, S>(U, g.npad, mu, s, m);
 }
 
+// Streaming copy of n 16-byte words: the reference point for achievable HBM
+// bandwidth and for calibrating byte counters.
+__global__ __launch_bounds__(kBlock) void k_stream_copy(const uint4* __restrict__ a, uint4* __restrict__ b,
+                                                         int64_t n)
+{
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) b[i] = a[i];
+}
+
+hipError_t launch_stream_copy(hipStream_t stream, const void* src, void* dst, size_t bytes)
+{
+    const int64_t n = (int64_t)(bytes / 16);
+    k_stream_copy<<<8192, kBlock, 0, stream>>>((const uint4*)src, (uint4*)dst, n);
+    return hipGetLastError();
+}
+
 static dim3 link_grid(const lqcd_lattice* L)
 {
     return dim3(grid_for(L->g.nsite), (unsigned)L->g.dim);
