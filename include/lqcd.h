@@ -174,6 +174,17 @@ typedef struct {
 int lqcd_metropolis_sweep(lqcd_lattice* lat, double beta, size_t n_update,
                           const double* proposals, const uint64_t* uniforms,
                           lqcd_mh_stats* stats);
+/* The same sweep with the randomness drawn on the device, replacing the
+ * rng argument of MetropolisHastingsSweep::next_element
+ * (metropolis_hastings_sweep.rs:146-174) and potential_modif (:126-143).
+ * Proposal j of link l is orthonormalize(random_su3_close_to_unity(spread))
+ * built from 14 uniforms of Philox4x32-10 blocks 64 + 8j + (0..6), the
+ * accept word from block 63; counter selects the sweep, key = seed.  The
+ * stream differs from the reference's xoshiro draws but every draw has the
+ * reference's distribution.  n_update <= 8000, 0 < spread < 1. */
+int lqcd_metropolis_sweep_random(lqcd_lattice* lat, double beta, size_t n_update,
+                                 double spread, uint64_t seed, uint64_t counter,
+                                 lqcd_mh_stats* stats);
 
 /* LatticeStateDefault::normalize_link_matrices (src/simulation/state.rs:754-756) */
 int lqcd_normalize_link_matrices(lqcd_lattice* lat);

@@ -585,18 +585,50 @@ int lqcd_wilson_backward(lqcd_lattice* L, double beta, double* grad, size_t n_li
     return force_to_host(L, grad, n_links, -(beta * L->size) / std::sqrt(6.0));
 }
 
-int lqcd_metropolis_sweep(lqcd_lattice* L, double beta, size_t n_update, const double* proposals,
-                          const uint64_t* uniforms, lqcd_mh_stats* stats)
+static int metropolis_prepare(lqcd_lattice* L, double beta, size_t n_update)
 {
-    CHECK_ARG(L && proposals && uniforms, "null argument");
     CHECK_ARG(n_update > 0, "n_update must be positive");
     CHECK_ARG(std::isfinite(beta), "beta must be finite");
     for (int i = 0; i < L->g.dim; ++i)
         if (L->g.extent[i] % 2)
             return fail(LQCD_ERR_UNSUPPORTED, "checkerboard sweep needs even extents");
-    DeviceGuard dg(L->ctx->device);
     if (!L->mh_p) HIP_TRY(hipMalloc((void**)&L->mh_p, sizeof(double) * L->nlink));
     if (!L->mh_acc) HIP_TRY(hipMalloc((void**)&L->mh_acc, sizeof(uint32_t) * L->nlink));
+    return LQCD_OK;
+}
+
+static int metropolis_finish(lqcd_lattice* L, lqcd_mh_stats* stats)
+{
+    HIP_TRY(launch_mh_stats(L, L->result));
+    if (int rc = read_results(L, 2)) return rc;
+    if (stats) {
+        stats->prob_replace_mean = L->host_result[0] / (double)L->nlink;
+        stats->number_replace_last = (size_t)L->host_result[1];
+    }
+    return LQCD_OK;
+}
+
+int lqcd_metropolis_sweep_random(lqcd_lattice* L, double beta, size_t n_update, double spread,
+                                 uint64_t seed, uint64_t counter, lqcd_mh_stats* stats)
+{
+    CHECK_ARG(L, "lattice is null");
+    CHECK_ARG(std::isfinite(spread) && spread > 0.0 && spread < 1.0, "spread must be in (0, 1)");
+    CHECK_ARG(n_update <= kMaxMhUpdates, "n_update exceeds the device proposal stream");
+    DeviceGuard dg(L->ctx->device);
+    if (int rc = metropolis_prepare(L, beta, n_update)) return rc;
+    for (int mu = 0; mu < L->g.dim; ++mu)
+        for (int par = 0; par < 2; ++par)
+            HIP_TRY(launch_metropolis_class_rng(L, L->U[L->cur], mu, par, beta, n_update, spread, seed,
+                                                counter));
+    return metropolis_finish(L, stats);
+}
+
+int lqcd_metropolis_sweep(lqcd_lattice* L, double beta, size_t n_update, const double* proposals,
+                          const uint64_t* uniforms, lqcd_mh_stats* stats)
+{
+    CHECK_ARG(L && proposals && uniforms, "null argument");
+    DeviceGuard dg(L->ctx->device);
+    if (int rc = metropolis_prepare(L, beta, n_update)) return rc;
     DevBuf dp, du;
     const size_t pbytes = sizeof(double) * 18 * L->nlink * n_update;
     HIP_TRY(hipMalloc(&dp.p, pbytes));
@@ -608,13 +640,7 @@ int lqcd_metropolis_sweep(lqcd_lattice* L, double beta, size_t n_update, const d
         for (int par = 0; par < 2; ++par)
             HIP_TRY(launch_metropolis_class(L, L->U[L->cur], mu, par, beta, n_update,
                                             (const double*)dp.p, (const uint64_t*)du.p));
-    HIP_TRY(launch_mh_stats(L, L->result));
-    if (int rc = read_results(L, 2)) return rc;
-    if (stats) {
-        stats->prob_replace_mean = L->host_result[0] / (double)L->nlink;
-        stats->number_replace_last = (size_t)L->host_result[1];
-    }
-    return LQCD_OK;
+    return metropolis_finish(L, stats);
 }
 
 int lqcd_normalize_link_matrices(lqcd_lattice* L)

@@ -49,6 +49,8 @@ namespace lqcd {
 constexpr int kBlock = 256;
 constexpr int kReduceSlots = 4;   // doubles per block partial
 constexpr int kMaxReduceBlocks = 4096;
+// device Metropolis proposals use Philox blocks 64 + 8 j + (0..6) < 2^16
+constexpr size_t kMaxMhUpdates = 8000;
 
 enum ProfileKind {
     PROF_KICK_DRIFT = 0,
@@ -105,6 +107,9 @@ hipError_t launch_drift(const lqcd_lattice* L, const void* U_in, void* U_out, co
 hipError_t launch_metropolis_class(const lqcd_lattice* L, void* U, int mu, int parity, double beta,
                                    size_t n_update, const double* d_proposals,
                                    const uint64_t* d_uniforms);
+// the same class update with proposals and accept words drawn on the device
+hipError_t launch_metropolis_class_rng(const lqcd_lattice* L, void* U, int mu, int parity, double beta,
+                                       size_t n_update, double spread, uint64_t seed, uint64_t counter);
 
 // profiling hooks (capi.cpp)
 void prof_begin(lqcd_ctx* ctx, int kind, lqcd_prof_event* ev);

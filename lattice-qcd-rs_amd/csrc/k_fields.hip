@@ -1,6 +1,7 @@
 // k_fields.hip -- layout conversion, cold start, device momenta, reunitarisation.
 #include "dispatch.hpp"
 #include "fields.hpp"
+#include "rng_device.hpp"
 
 namespace lqcd {
 
@@ -86,25 +87,6 @@ __global__ __launch_bounds__(kBlock) void k_efield_zero(S* __restrict__ E, Geom 
     if (s >= g.nsite) return;
     double e[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     store_e<double, S>(E, g.npad, mu, s, e);
-}
-
-// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11)
-__device__ __forceinline__ uint4 philox4x32(uint4 c, uint2 k)
-{
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        const uint32_t lo0 = 0xD2511F53u * c.x, hi0 = __umulhi(0xD2511F53u, c.x);
-        const uint32_t lo1 = 0xCD9E8D57u * c.z, hi1 = __umulhi(0xCD9E8D57u, c.z);
-        c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
-        k.x += 0x9E3779B9u;
-        k.y += 0xBB67AE85u;
-    }
-    return c;
-}
-
-__device__ __forceinline__ double u53(uint32_t a, uint32_t b)
-{
-    return (double)(((uint64_t)(a >> 5) << 26) | (uint64_t)(b >> 6)) * 0x1.0p-53;
 }
 
 // Gaussian momenta, e^a_mu(x) ~ N(0, sigma): link l = site*D + mu, block j

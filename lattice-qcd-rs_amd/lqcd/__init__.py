@@ -473,16 +473,36 @@ class HybridMonteCarloDiagnostic:
 
 class MetropolisHastingsSweep:
     """``MetropolisHastingsSweep`` (src/simulation/monte_carlo/metropolis_hastings_sweep.rs),
-    checkerboard order on the device with host-given proposals and uniforms."""
+    checkerboard order on the device.  ``sweep`` takes host-given proposals and
+    uniforms; ``next_element`` draws them on the device from a Philox stream
+    keyed by ``seed`` (the reference's ``rng`` argument), one counter per sweep."""
 
-    def __init__(self, number_of_update: int, spread: float):
+    def __init__(self, number_of_update: int, spread: float, seed: int = 0):
         # MetropolisHastingsSweep::new returns None for these (:77-88)
         if number_of_update == 0 or not (0.0 < spread < 1.0):
             raise ValueError("number_of_update must be > 0 and spread in (0, 1)")
         self._n_update = int(number_of_update)
         self._spread = float(spread)
+        self._seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+        self._counter = 0
         self._prob_replace_mean = 0.0
         self._number_replace_last = 0
+
+    def counter(self) -> int:
+        """Sweeps drawn so far from the device stream."""
+        return self._counter
+
+    def next_element(self, state: LatticeStateDefault) -> LatticeStateDefault:
+        """One sweep with proposals and accept draws generated on the device
+        (metropolis_hastings_sweep.rs:146-174)."""
+        st = F.MhStats()
+        _raise(F.lib().lqcd_metropolis_sweep_random(state.handle, state.beta(), self._n_update,
+                                                    self._spread, self._seed, self._counter,
+                                                    C.byref(st)))
+        self._counter += 1
+        self._prob_replace_mean = st.prob_replace_mean
+        self._number_replace_last = int(st.number_replace_last)
+        return state
 
     def number_of_update(self) -> int:
         return self._n_update

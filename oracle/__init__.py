@@ -119,6 +119,7 @@ def _declare(L):
         "or_philox4x32": (None, [_P, _P, _P]),
         "or_device_momenta": (None, [LP, _U64, _U64, _D, _P]),
         "or_device_hot_start": (None, [LP, _U64, _P]),
+        "or_device_mh_inputs": (None, [LP, _U64, _U64, _D, _S, _P, _P]),
         "or_bernoulli_accept": (_I, [_D, _U64]),
         "or_clamp_probability": (_D, [_D]),
         "or_link_matrix": (None, [LP, _P, _S, _I, _I, _P]),
@@ -245,6 +246,14 @@ class OracleLattice:
         e = np.zeros((self.nlink, 8))
         lib().or_device_momenta(self.ref, seed, counter, sigma, _ptr(e))
         return e
+
+    def device_mh_inputs(self, seed: int, counter: int, spread: float, n_update: int = 1):
+        """Proposals (n_update, nlink, 18) and accept words (nlink,) of the
+        device Metropolis stream."""
+        p = np.zeros((n_update, self.nlink, 18))
+        u = np.zeros(self.nlink, dtype=np.uint64)
+        lib().or_device_mh_inputs(self.ref, seed, counter, spread, n_update, _ptr(p), _ptr(u))
+        return p, u
 
     def mh_proposals(self, seed: int, spread: float, n_update: int = 1) -> np.ndarray:
         p = np.zeros((n_update, self.nlink, 18))

@@ -573,16 +573,14 @@ void or_random_su3(or_rng* r, or_mat3* out)
  * complex_matrix_from_vec :134-140 and PAULI_* :14-49.  PAULI_3 is the
  * identity in the reference (src/su2.rs:39-45); that is restated as is.
  * out = row-major 2x2 [m00, m01, m10, m11]. */
-void or_random_su2_close_to_unity(or_rng* r, double spread, or_cplx out[4])
+static void su2_close_to_unity_from(const double v[3], int positive, double spread, or_cplx out[4])
 {
-    double v[3];
-    for (int i = 0; i < 3; ++i) v[i] = or_rng_uniform(r, -1.0, 1.0);
     double n = sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
     double x[3];
     for (int i = 0; i < 3; ++i) x[i] = (n <= F64_EPS ? v[i] : v[i] / n) * spread;
     double nsq = (x[0] * x[0] + x[1] * x[1]) + x[2] * x[2];
     double x0u = sqrt(1.0 - nsq);
-    double x0 = or_rng_bernoulli(r, 0.5) ? x0u : -x0u;
+    double x0 = positive ? x0u : -x0u;
     /* identity * x0 + sum_i PAULI_i * (0 + i x_i) */
     or_cplx m[4] = {cx(x0, 0), C_ZERO, C_ZERO, cx(x0, 0)};
     /* PAULI_1 = [[0,1],[1,0]] */
@@ -601,13 +599,18 @@ void or_random_su2_close_to_unity(or_rng* r, double spread, or_cplx out[4])
     for (int k = 0; k < 4; ++k) out[k] = cadd(m[k], p[k]);
 }
 
-/* random_su3_close_to_unity, src/su3.rs:385-398, with get_r/s/t :428-530 */
-void or_random_su3_close_to_unity(or_rng* r, double spread, or_mat3* out)
+void or_random_su2_close_to_unity(or_rng* r, double spread, or_cplx out[4])
 {
-    or_cplx a[4], b[4], c[4];
-    or_random_su2_close_to_unity(r, spread, a);
-    or_random_su2_close_to_unity(r, spread, b);
-    or_random_su2_close_to_unity(r, spread, c);
+    double v[3];
+    for (int i = 0; i < 3; ++i) v[i] = or_rng_uniform(r, -1.0, 1.0);
+    int positive = or_rng_bernoulli(r, 0.5);
+    su2_close_to_unity_from(v, positive, spread, out);
+}
+
+/* get_r(a) get_s(b) get_t(c), adjoint when adj (src/su3.rs:385-398, :428-530) */
+static void su3_from_su2(const or_cplx a[4], const or_cplx b[4], const or_cplx c[4], int adj,
+                         or_mat3* out)
+{
     or_mat3 R, S, T, RS, X;
     mzero(&R); mzero(&S); mzero(&T);
     M(&R, 0, 0) = a[0]; M(&R, 0, 1) = a[1]; M(&R, 1, 0) = a[2]; M(&R, 1, 1) = a[3];
@@ -618,8 +621,18 @@ void or_random_su3_close_to_unity(or_rng* r, double spread, or_mat3* out)
     M(&T, 1, 1) = c[0]; M(&T, 1, 2) = c[1]; M(&T, 2, 1) = c[2]; M(&T, 2, 2) = c[3];
     or_matmul(&R, &S, &RS);
     or_matmul(&RS, &T, &X);
-    if (or_rng_bernoulli(r, 0.5)) or_adj(&X, &X);
+    if (adj) or_adj(&X, &X);
     *out = X;
+}
+
+/* random_su3_close_to_unity, src/su3.rs:385-398, with get_r/s/t :428-530 */
+void or_random_su3_close_to_unity(or_rng* r, double spread, or_mat3* out)
+{
+    or_cplx a[4], b[4], c[4];
+    or_random_su2_close_to_unity(r, spread, a);
+    or_random_su2_close_to_unity(r, spread, b);
+    or_random_su2_close_to_unity(r, spread, c);
+    su3_from_su2(a, b, c, or_rng_bernoulli(r, 0.5), out);
 }
 
 /* LinkMatrix::new_determinist, src/field.rs:646-659 (storage order) */
@@ -750,6 +763,51 @@ void or_device_hot_start(const or_lattice* l, uint64_t seed, double* links)
         or_mat3 m;
         ortho_matrix_from_2_vector(&v1, &v2, &m);
         store_mat(links + 18 * li, &m);
+    }
+}
+
+/* Device Metropolis stream (the rng of MetropolisHastingsSweep::next_element,
+ * metropolis_hastings_sweep.rs:146-174, drawn per link): Philox counter
+ * {lo32(l), hi16(l) | blk << 16, lo32(counter), hi32(counter)}, key = seed.
+ * Update j of link l uses blocks 64 + 8j + b (b = 0..6): 14 uniforms
+ * w0..w13 = u53 pairs.  su2 number t (t = 0, 1, 2) takes v = 2 w_{4t+k} - 1
+ * (k = 0..2) as the Uniform(-1, 1) draws and w_{4t+3} < 0.5 as its
+ * Bernoulli(0.5); w12 < 0.5 is the adjoint Bernoulli; the proposal is
+ * orthonormalize(get_r get_s get_t) as potential_modif (:126-143) forms it.
+ * The accept word of link l is (x << 32) | y of block 63.
+ * proposals [n_update][link][18] column-major, uniforms [link]. */
+void or_device_mh_inputs(const or_lattice* l, uint64_t seed, uint64_t counter, double spread,
+                         size_t n_update, double* proposals, uint64_t* uniforms)
+{
+    size_t nl = l->nsite * (size_t)l->dim;
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    for (size_t li = 0; li < nl; ++li) {
+        uint32_t hi = (uint32_t)((li >> 32) & 0xFFFFu);
+        for (size_t j = 0; j < n_update; ++j) {
+            double w[14];
+            for (uint32_t b = 0; b < 7; ++b) {
+                uint32_t blk = 64u + 8u * (uint32_t)j + b;
+                uint32_t ctr[4] = {(uint32_t)li, hi | (blk << 16), (uint32_t)counter,
+                                   (uint32_t)(counter >> 32)};
+                uint32_t o[4];
+                or_philox4x32(ctr, key, o);
+                w[2 * b] = u53(o[0], o[1]);
+                w[2 * b + 1] = u53(o[2], o[3]);
+            }
+            or_cplx su2[3][4];
+            for (int t = 0; t < 3; ++t) {
+                double v[3] = {2.0 * w[4 * t] - 1.0, 2.0 * w[4 * t + 1] - 1.0, 2.0 * w[4 * t + 2] - 1.0};
+                su2_close_to_unity_from(v, w[4 * t + 3] < 0.5, spread, su2[t]);
+            }
+            or_mat3 m, o;
+            su3_from_su2(su2[0], su2[1], su2[2], w[12] < 0.5, &m);
+            or_orthonormalize(&m, &o);
+            store_mat(proposals + 18 * (j * nl + li), &o);
+        }
+        uint32_t ctr[4] = {(uint32_t)li, hi | (63u << 16), (uint32_t)counter, (uint32_t)(counter >> 32)};
+        uint32_t o[4];
+        or_philox4x32(ctr, key, o);
+        uniforms[li] = ((uint64_t)o[0] << 32) | (uint64_t)o[1];
     }
 }
 

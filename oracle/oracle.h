@@ -82,6 +82,8 @@ void     or_mh_proposals(const or_lattice* l, uint64_t seed, double spread,
 /* counter-based stream used by the device-side generators */
 void     or_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 void     or_device_hot_start(const or_lattice* l, uint64_t seed, double* links);
+void     or_device_mh_inputs(const or_lattice* l, uint64_t seed, uint64_t counter, double spread,
+                             size_t n_update, double* proposals, uint64_t* uniforms);
 void     or_device_momenta(const or_lattice* l, uint64_t seed, uint64_t counter,
                            double sigma, double* e);
 

@@ -208,6 +208,50 @@ def test_metropolis_sweep(dim, ext, n_update):
         assert np.max(np.abs(st.link_matrix() - u)) <= 1e-12
 
 
+@pytest.mark.parametrize("dim,ext,n_update,precision", [
+    (4, [4, 4, 4, 6], 1, lqcd.F64),
+    (4, [4, 4, 4, 4], 3, lqcd.F64),
+    (3, [6, 4, 8], 2, lqcd.F64),
+    (4, [4, 4, 4, 6], 2, lqcd.F32),
+])
+def test_metropolis_device_stream(dim, ext, n_update, precision):
+    announce(f"metropolis device stream D={dim} ext={ext} n_update={n_update} prec={precision}")
+    beta, spread, seed = 5.5, 0.2, 0x5EED1234ABCD
+    ol, u, st = make_state(dim, ext, beta, precision=precision)
+    u = st.link_matrix()  # f32 storage: start the oracle from the stored values
+    mh = lqcd.MetropolisHastingsSweep(n_update, spread, seed)
+    for sweep in range(3):
+        props, unis = ol.device_mh_inputs(seed, sweep, spread, n_update)
+        mh.next_element(st)
+        u, ref = ol.metropolis_sweep(u, beta, props, unis, O.ORDER_CHECKERBOARD)
+        if precision == lqcd.F64:
+            assert mh.number_replace_last() == ref.number_replace_last
+            assert abs(mh.prob_replace_mean() - ref.prob_replace_mean) <= 1e-12
+            assert np.max(np.abs(st.link_matrix() - u)) <= 1e-12
+        else:
+            # accepted links are stored f32; the oracle continues from them
+            assert abs(mh.number_replace_last() - ref.number_replace_last) <= 2
+            assert abs(mh.prob_replace_mean() - ref.prob_replace_mean) <= 1e-5
+            got = st.link_matrix()
+            same = np.max(np.abs(got - u), axis=1) <= 1e-6
+            assert same.mean() >= 0.999
+            u = got
+    assert mh.counter() == 3
+    assert 0 < mh.number_replace_last() < ol.nlink
+
+
+def test_metropolis_device_stream_rejects_bad_arguments():
+    announce("metropolis device stream arguments")
+    _, _, st = make_state(4, [4, 4, 4, 4])
+    sweep = F.lib().lqcd_metropolis_sweep_random
+    assert sweep(st.handle, 6.0, 8001, 0.1, 1, 0, None) == F.ERR_INVALID_ARGUMENT
+    assert sweep(st.handle, 6.0, 1, 1.5, 1, 0, None) == F.ERR_INVALID_ARGUMENT
+    assert sweep(st.handle, 6.0, 0, 0.1, 1, 0, None) == F.ERR_INVALID_ARGUMENT
+    assert sweep(st.handle, 6.0, 1, 0.1, 1, 0, None) == F.OK
+    with pytest.raises(ValueError):
+        lqcd.MetropolisHastingsSweep(1, 1.0, 3)
+
+
 def test_metropolis_rejects_odd_extent():
     announce("metropolis odd extent")
     lat = lqcd.LatticeCyclic(1.0, 3, 3)
@@ -270,3 +314,46 @@ def test_f32_path_errors_vs_f64_oracle():
                                                 0.01, 10, drift, O.SCHEME_LEAPFROG)
         assert np.max(np.abs(ef.link_matrix() - u_ref)) <= 1e-5
         assert np.max(np.abs(ef.e_field() - e_ref)) <= 1e-5
+
+
+def _perturbed_action(st, ef, u, omega, eps):
+    """S_W(exp(i eps Omega) U) on the device: Omega as E, one exp drift of
+    dt = eps a / sqrt(6) (U' = exp(i dt sqrt(6)/a E) U)."""
+    st.set_link_matrix(u)
+    ef.set_e_field(omega)
+    lqcd.SymplecticEulerGpu(lqcd.DRIFT_EXP).integrate_link(ef, eps * st.lattice().size() / math.sqrt(6))
+    return st.wilson_forward()
+
+
+@pytest.mark.parametrize("precision", [lqcd.F64])
+def test_wilson_adjoint_finite_differences_16x4(precision):
+    # adjoint check: D=4 16^4, hot start, beta=5.7; backward vs central finite
+    # differences of the forward (eps=1e-4, 64 random directions, Gaussian
+    # coefficients) and vs the oracle's derivative_e
+    announce("adjoint 16^4 finite differences")
+    beta = 5.7
+    ol = O.OracleLattice(4, 16)
+    u = ol.device_hot_start(0x5EED)
+    st = lqcd.LatticeStateDefault(lqcd.LatticeCyclic(1.0, 16, 4), beta, u, precision)
+    ef = lqcd.LatticeStateEFSyncDefault(st)
+    grad = st.wilson_backward()
+    # backward == -(beta a / sqrt 6) derivative_e of the oracle
+    ref = -(beta / math.sqrt(6)) * ol.derivative_e(u)
+    assert np.max(np.abs(grad - ref)) <= 1e-12 * np.max(np.abs(ref))
+    rng = np.random.default_rng(64)
+    worst = 0.0
+    for k in range(64):
+        omega = rng.normal(size=(ol.nlink, 8))
+        pred = float(np.sum(omega * grad))
+        fd = (_perturbed_action(st, ef, u, omega, 1e-4)
+              - _perturbed_action(st, ef, u, omega, -1e-4)) / 2e-4
+        worst = max(worst, abs(fd - pred) / abs(pred))
+        if k < 4:
+            # Richardson extrapolation removes the eps^2 term
+            f1 = (_perturbed_action(st, ef, u, omega, 1e-3)
+                  - _perturbed_action(st, ef, u, omega, -1e-3)) / 2e-3
+            f2 = (_perturbed_action(st, ef, u, omega, 2e-3)
+                  - _perturbed_action(st, ef, u, omega, -2e-3)) / 4e-3
+            assert abs((4 * f1 - f2) / 3 - pred) <= 1e-7 * abs(pred)
+    print(f"adjoint: worst relative |fd - <Omega, grad>| over 64 directions = {worst:.2e}")
+    assert worst <= 1e-5

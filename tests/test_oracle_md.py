@@ -198,6 +198,42 @@ def test_device_momenta_statistics():
     assert not np.allclose(e, e2)
 
 
+def test_device_mh_stream_matches_reference_distribution():
+    # The device Metropolis stream must draw orthonormalize(random_su3_close_to_unity)
+    # (src/su3.rs:385-398) with the reference's distribution: every proposal is SU(3),
+    # and the mean of each real component over 16384 proposals agrees with the
+    # xoshiro-driven restatement within 5 standard errors (a dropped 2w-1 shift
+    # moves one by ~40 sigma).  The Bernoulli(0.5) adjoint makes E|X_ij|^2 = E|X_ji|^2
+    # exactly; dropping it breaks that by ~20 sigma at this sample size.
+    l = O.OracleLattice(4, 4)
+    n_update, spread = 16, 0.3
+    p, w = l.device_mh_inputs(0xC0FFEE, 5, spread, n_update)
+    q = l.mh_proposals(0xC0FFEE, spread, n_update)
+    pm = p.reshape(-1, 18)
+    qm = q.reshape(-1, 18)
+    mats = O.to_np(pm)
+    eye = np.eye(3)
+    assert np.max(np.abs(mats @ np.conj(np.swapaxes(mats, -1, -2)) - eye)) < 1e-13
+    assert np.max(np.abs(np.linalg.det(mats) - 1.0)) < 1e-13
+    err = np.sqrt(np.var(pm, axis=0) / len(pm) + np.var(qm, axis=0) / len(qm)) + 1e-12
+    assert np.max(np.abs(pm.mean(axis=0) - qm.mean(axis=0)) / err) < 5.0
+    assert abs(np.var(pm) - np.var(qm)) < 0.05 * np.var(qm)
+    a2 = np.abs(mats) ** 2
+    d = (a2[:, 1, 0] + a2[:, 2, 1] + a2[:, 0, 2]) - (a2[:, 0, 1] + a2[:, 1, 2] + a2[:, 2, 0])
+    assert abs(d.mean()) < 5.0 * d.std() / math.sqrt(len(d))
+    # accept words uniform on [0, 2^64)
+    frac = w.astype(np.float64) / 2.0**64
+    assert abs(frac.mean() - 0.5) < 5.0 / math.sqrt(12 * len(frac))
+    # reproducible per (seed, counter); a new counter is a new stream
+    p2, w2 = l.device_mh_inputs(0xC0FFEE, 5, spread, n_update)
+    assert np.array_equal(p, p2) and np.array_equal(w, w2)
+    p3, w3 = l.device_mh_inputs(0xC0FFEE, 6, spread, n_update)
+    assert not np.allclose(p, p3) and not np.array_equal(w, w3)
+    # update j of a link does not depend on how many updates are drawn
+    p1, _ = l.device_mh_inputs(0xC0FFEE, 5, spread, 1)
+    assert np.array_equal(p1[0], p[0])
+
+
 def test_parallel_first_call_matches_serial(tmp_path):
     # regression: the generator tables are initialised on first use, which
     # happens inside OpenMP loops; a fresh process with many threads must give
