@@ -204,6 +204,76 @@ int lqcd_md_autotune(lqcd_lattice* lat, int drift, int* variant);
 int lqcd_md_variant_set(lqcd_lattice* lat, int variant);
 int lqcd_md_variant_get(const lqcd_lattice* lat, int* variant);
 
+/* ------------------------------------------------------ batched chains */
+/* Many independent chains, each a LatticeStateDefault of its own extents,
+ * beta and seed advanced by its own HybridMonteCarloDiagnostic
+ * (src/simulation/monte_carlo/hybrid_monte_carlo.rs:465-471, 584-613), kept
+ * in one set of device buffers and advanced together: one launch per MD
+ * step covers every chain.  The reference runs such chains one after the
+ * other; there is no batch type in it.  All chains share D and precision. */
+typedef struct lqcd_batch lqcd_batch;
+
+typedef struct {
+    size_t extent[4];  /* the first D entries are used, each >= 2 */
+    double beta;
+    double size;       /* lattice spacing a > 0 */
+    uint64_t seed;     /* hot start, momenta and accept streams of the chain */
+} lqcd_chain_spec;
+
+typedef struct {
+    double h_old;        /* H before the trajectory (hamiltonian_total) */
+    double h_new;        /* H at its end */
+    double delta_h;      /* h_new - h_old */
+    double probability;  /* probability_of_replacement */
+    double plaquette;    /* Re average_trace_plaquette / 3 of the state kept */
+    int32_t accepted;
+    int32_t reserved;
+} lqcd_chain_stats;
+
+/* LatticeCyclic::new / LatticeStateDefault::new_cold per chain (error codes
+ * as lqcd_lattice_create); every chain starts cold. */
+int lqcd_batch_create(lqcd_ctx* ctx, int dim, int precision, const lqcd_chain_spec* chains,
+                      size_t nchain, lqcd_batch** out);
+int lqcd_batch_destroy(lqcd_batch* batch);
+int lqcd_batch_info(const lqcd_batch* batch, size_t* nchain, size_t* nsite_total);
+int lqcd_batch_chain_info(const lqcd_batch* batch, size_t chain, size_t* nsite, size_t* nlink);
+/* LinkMatrix::new_cold (src/field.rs:716-722) for every chain */
+int lqcd_batch_cold_start(lqcd_batch* batch);
+/* every chain drawn as lqcd_links_random(seed of the chain) */
+int lqcd_batch_hot_start(lqcd_batch* batch);
+/* LatticeState::link_matrix / set_link_matrix of one chain */
+int lqcd_batch_links_upload(lqcd_batch* batch, size_t chain, const double* links, size_t n_links);
+int lqcd_batch_links_download(lqcd_batch* batch, size_t chain, double* links, size_t n_links);
+/* Re average_trace_plaquette / 3 (src/field.rs:775-804) of every chain */
+int lqcd_batch_plaquette(lqcd_batch* batch, double* plaquette);
+/* The trajectory counter n of the random streams: trajectory n of chain c
+ * draws its momenta as lqcd_efield_random(seed_c, n, 0.5 / beta_c) and its
+ * accept word as (x << 32) | y of Philox4x32-10 counter {0, 62 << 16,
+ * lo32(n), hi32(n)}, key seed_c.  Starts at 0, +1 per trajectory. */
+int lqcd_batch_counter_get(const lqcd_batch* batch, uint64_t* counter);
+int lqcd_batch_counter_set(lqcd_batch* batch, uint64_t counter);
+/* n_traj HMC trajectories of every chain: momenta refresh, H, n_steps of
+ * the scheme with step dt, H, accept on the device, rejected chains keep
+ * their links; reunitarize != 0 then applies normalize_link_matrices
+ * (src/simulation/state.rs:754-756).  stats: [n_traj][nchain] or NULL.
+ * One trajectory is captured once as a HIP graph and replayed (see
+ * lqcd_batch_graph_enable). */
+int lqcd_batch_hmc(lqcd_batch* batch, double dt, size_t n_steps, int drift, int scheme,
+                   size_t n_traj, int reunitarize, lqcd_chain_stats* stats);
+/* Launch shape of the batched MD kernel, config in 0..11:
+ *   config % 3        workgroup of 64 / 128 / 256 threads
+ *   (config / 3) % 2  1: one thread per site for all D links; 0: per link
+ *   config / 6        1: persistent grid (resident workgroups walk tile
+ *                     ranges); 0: one workgroup per 1-4 tiles
+ * Shapes differ only in rounding (staple summation order).  The first
+ * lqcd_batch_hmc autotunes unless a config was set or LQCD_BATCH_CONFIG is
+ * set in the environment. */
+int lqcd_batch_autotune(lqcd_batch* batch, int drift, int* config);
+int lqcd_batch_config_set(lqcd_batch* batch, int config);
+int lqcd_batch_config_get(const lqcd_batch* batch, int* config);
+/* 1 (default): replay a captured graph per trajectory; 0: launch eagerly */
+int lqcd_batch_graph_enable(lqcd_batch* batch, int enable);
+
 /* ------------------------------------------------------------ profiling */
 /* Kernel kinds for lqcd_profile_read (HIP events around each launch). */
 typedef enum {

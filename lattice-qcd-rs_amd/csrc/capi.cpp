@@ -12,45 +12,18 @@
 #include <string>
 
 #include "../../include/lqcd.h"
+#include "capi_util.hpp"
 #include "internal.hpp"
 
 using namespace lqcd;
 
-namespace {
-
-thread_local std::string g_last_error;
-
-int fail(int code, const std::string& msg)
+std::string& lqcd::last_error_slot()
 {
-    g_last_error = msg;
-    return code;
+    thread_local std::string msg;
+    return msg;
 }
 
-#define HIP_TRY(expr)                                                                  \
-    do {                                                                               \
-        hipError_t _e = (expr);                                                        \
-        if (_e != hipSuccess)                                                          \
-            return fail(_e == hipErrorOutOfMemory ? LQCD_ERR_OUT_OF_MEMORY : LQCD_ERR_DEVICE, \
-                        std::string(#expr) + ": " + hipGetErrorString(_e));            \
-    } while (0)
-
-#define CHECK_ARG(cond, msg)                                                           \
-    do {                                                                               \
-        if (!(cond)) return fail(LQCD_ERR_INVALID_ARGUMENT, msg);                      \
-    } while (0)
-
-struct DeviceGuard {
-    explicit DeviceGuard(int dev) { (void)hipSetDevice(dev); }
-};
-
-// Temporary device buffer freed on scope exit.
-struct DevBuf {
-    void* p = nullptr;
-    ~DevBuf()
-    {
-        if (p) (void)hipFree(p);
-    }
-};
+namespace {
 
 int other_buffers(int cur, int* a, int* b)
 {
@@ -208,7 +181,7 @@ void prof_end(lqcd_ctx* ctx, lqcd_prof_event* ev)
 
 extern "C" {
 
-const char* lqcd_last_error(void) { return g_last_error.c_str(); }
+const char* lqcd_last_error(void) { return last_error_slot().c_str(); }
 
 const char* lqcd_version(void) { return "lqcd-mi355x 0.1.0 (gfx950)"; }
 

@@ -1,7 +1,7 @@
 // k_fields.hip -- layout conversion, cold start, device momenta, reunitarisation.
 #include "dispatch.hpp"
 #include "fields.hpp"
-#include "rng_device.hpp"
+#include "generators.hpp"
 
 namespace lqcd {
 
@@ -89,9 +89,7 @@ __global__ __launch_bounds__(kBlock) void k_efield_zero(S* __restrict__ E, Geom 
     store_e<double, S>(E, g.npad, mu, s, e);
 }
 
-// Gaussian momenta, e^a_mu(x) ~ N(0, sigma): link l = site*D + mu, block j
-// (0..3) uses counter {lo32(l), hi16(l) | j << 16, lo32(n), hi32(n)} and key =
-// seed; each block is one Box-Muller pair.
+// Gaussian momenta e^a_mu(x) ~ N(0, sigma) (momenta_link), link l = site*D + mu.
 template <int D, typename S>
 __global__ __launch_bounds__(kBlock) void k_device_momenta(S* __restrict__ E, Geom g, uint64_t seed,
                                                            uint64_t counter, double sigma)
@@ -99,69 +97,20 @@ __global__ __launch_bounds__(kBlock) void k_device_momenta(S* __restrict__ E, Ge
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int mu = blockIdx.y;
     if (s >= g.nsite) return;
-    const uint64_t l = (uint64_t)s * D + mu;
-    const uint2 key = make_uint2((uint32_t)seed, (uint32_t)(seed >> 32));
     double e[8];
-#pragma unroll
-    for (uint32_t j = 0; j < 4; ++j) {
-        const uint4 ctr = make_uint4((uint32_t)l, (uint32_t)((l >> 32) & 0xFFFFu) | (j << 16),
-                                     (uint32_t)counter, (uint32_t)(counter >> 32));
-        const uint4 o = philox4x32(ctr, key);
-        const double u = u53(o.x, o.y);
-        const double w = u53(o.z, o.w);
-        const double rad = sqrt(-2.0 * log(1.0 - u)) * sigma;
-        double sn, cs;
-        sincospi(2.0 * w, &sn, &cs);
-        e[2 * j] = rad * cs;
-        e[2 * j + 1] = rad * sn;
-    }
+    momenta_link((uint64_t)s * D + mu, seed, counter, sigma, e);
     store_e<double, S>(E, g.npad, mu, s, e);
 }
 
-// Hot start on the device: random_su3 (src/su3.rs:322-347) -- two complex
-// 3-vectors with re, im ~ U[-1, 1), redrawn while |v1| <= eps or
-// |v1 . v2| <= eps (non-conjugated dot), then the column Gram-Schmidt of
-// orthonormalize_matrix.  The 12 uniforms of attempt r come from the Philox
-// blocks j = 16 + 6 r .. 21 + 6 r of link l (counter as for the momenta,
-// counter word 2..3 = 0); at most 4 attempts.
+// Hot start on the device (random_su3_link), link l = site * D + mu.
 template <int D, typename S>
 __global__ __launch_bounds__(kBlock) void k_links_random(vec2_t<S>* __restrict__ U, Geom g, uint64_t seed)
 {
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int mu = blockIdx.y;
     if (s >= g.nsite) return;
-    const uint64_t l = (uint64_t)s * D + mu;
-    const uint2 key = make_uint2((uint32_t)seed, (uint32_t)(seed >> 32));
-    const double eps = 2.220446049250313e-16;
     mat3<double> m;
-    for (int attempt = 0; attempt < 4; ++attempt) {
-        double v[12];
-#pragma unroll
-        for (uint32_t j = 0; j < 6; ++j) {
-            const uint32_t blk = 16u + 6u * (uint32_t)attempt + j;
-            const uint4 ctr = make_uint4((uint32_t)l, (uint32_t)((l >> 32) & 0xFFFFu) | (blk << 16), 0u, 0u);
-            const uint4 o = philox4x32(ctr, key);
-            v[2 * j] = 2.0 * u53(o.x, o.y) - 1.0;
-            v[2 * j + 1] = 2.0 * u53(o.z, o.w) - 1.0;
-        }
-        // v1 = (v0 + i v1, v2 + i v3, v4 + i v5), v2 = (v6 + i v7, ...)
-#pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            m.e[r * 3 + 0] = {v[2 * r], v[2 * r + 1]};
-            m.e[r * 3 + 1] = {v[6 + 2 * r], v[6 + 2 * r + 1]};
-            m.e[r * 3 + 2] = {0.0, 0.0};
-        }
-        double n2 = 0.0, dre = 0.0, dim_ = 0.0;
-#pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            const cplx<double> a = m.e[r * 3 + 0], b = m.e[r * 3 + 1];
-            n2 += a.re * a.re + a.im * a.im;
-            dre += a.re * b.re - a.im * b.im;
-            dim_ += a.re * b.im + a.im * b.re;
-        }
-        if (sqrt(n2) > eps && sqrt(dre * dre + dim_ * dim_) > eps) break;
-    }
-    orthonormalize(m);
+    random_su3_link((uint64_t)s * D + mu, seed, m);
     store_link<double, S>(U, g.npad, mu, s, m);
 }
 

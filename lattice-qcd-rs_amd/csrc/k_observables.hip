@@ -6,62 +6,19 @@
 // the per-block partials in block order.  The result is bitwise reproducible
 // run to run for a given lattice shape.
 #include "dispatch.hpp"
-#include "fields.hpp"
+#include "observables_device.hpp"
 
 namespace lqcd {
 
-template <int K>
-__device__ __forceinline__ void block_reduce_to(double (&v)[K], double* __restrict__ dst)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-        for (int k = 0; k < K; ++k) v[k] += __shfl_xor(v[k], off, 64);
-    __shared__ double sh[kBlock / 64][K];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0)
-#pragma unroll
-        for (int k = 0; k < K; ++k) sh[wave][k] = v[k];
-    __syncthreads();
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            double t = 0.0;
-#pragma unroll
-            for (int w = 0; w < kBlock / 64; ++w) t += sh[w][k];
-            dst[k] = t;
-        }
-}
-
-// Per site: sum_{i<j} Tr P_ij(x) and sum_{i<j} (1 - Re Tr P_ij(x) / 3), with
-// P_ij(x) = U_i(x) U_j(x+i) U_i(x+j)^+ U_j(x)^+ (LinkMatrix::pij,
-// src/field.rs:761-771), as Tr[(U_i(x) U_j(x+i)) (U_j(x) U_i(x+j))^+].
+// plaquette_site summed over the lattice
 template <int D, typename S>
 __global__ __launch_bounds__(kBlock) void k_plaquette_partials(const vec2_t<S>* __restrict__ U, Geom g,
                                                                double* __restrict__ partials)
 {
     double acc[3] = {0.0, 0.0, 0.0};
     const int64_t step = (int64_t)gridDim.x * kBlock;
-    for (int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x; s < g.nsite; s += step) {
-        const Neighbours<D> nb = neighbours<D>(g, s);
-#pragma unroll
-        for (int i = 0; i < D; ++i) {
-            const mat3<double> ui = load_link<double, S>(U, g.npad, i, s);
-#pragma unroll
-            for (int j = i + 1; j < D; ++j) {
-                const mat3<double> uj = load_link<double, S>(U, g.npad, j, s);
-                const mat3<double> uj_pi = load_link<double, S>(U, g.npad, j, s + nb.up[i]);
-                const mat3<double> ui_pj = load_link<double, S>(U, g.npad, i, s + nb.up[j]);
-                mat3<double> a, b;
-                mul<false, false>(ui, uj_pi, a);
-                mul<false, false>(uj, ui_pj, b);
-                const cplx<double> tr = trace_mul<false, true>(a, b);
-                acc[0] += tr.re;
-                acc[1] += tr.im;
-                acc[2] += 1.0 - tr.re / 3.0;
-            }
-        }
-    }
+    for (int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x; s < g.nsite; s += step)
+        plaquette_site<D, S>(U, g, s, acc);
     block_reduce_to<3>(acc, partials + (int64_t)blockIdx.x * kReduceSlots);
 }
 
@@ -74,14 +31,7 @@ __global__ __launch_bounds__(kBlock) void k_efield_partials(const S* __restrict_
     double acc[1] = {0.0};
     const int64_t step = (int64_t)gridDim.x * kBlock;
     for (int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x; s < g.nsite; s += step)
-        for (int mu = 0; mu < g.dim; ++mu) {
-            double e[8];
-            load_e<double, S>(E, g.npad, mu, s, e);
-            double t = 0.0;
-#pragma unroll
-            for (int a = 0; a < 8; ++a) t += e[a] * e[a];
-            acc[0] += t / 2.0;
-        }
+        for (int mu = 0; mu < g.dim; ++mu) acc[0] += efield_energy_link<S>(E, g, mu, s);
     block_reduce_to<1>(acc, partials + (int64_t)blockIdx.x * kReduceSlots);
 }
 

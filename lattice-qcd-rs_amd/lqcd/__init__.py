@@ -13,6 +13,7 @@ reference (lattice-qcd-rs)              here
 ``SymplecticEulerRayon``                ``SymplecticEulerGpu``
 ``HybridMonteCarloDiagnostic``          ``HybridMonteCarloDiagnostic``
 ``MetropolisHastingsSweep``             ``MetropolisHastingsSweep`` (checkerboard)
+independent chains, one after another   ``BatchedChains`` (one launch per MD step)
 =====================================  ==========================================
 
 Host arrays use the reference's layouts: links ``(nsite*D, 18)`` float64,
@@ -33,6 +34,7 @@ from ._ffi import (DRIFT_ADDITIVE, DRIFT_EXP, F32, F64, SCHEME_LEAPFROG, SCHEME_
 __all__ = [
     "Context", "LatticeCyclic", "LatticeStateDefault", "LatticeStateEFSyncDefault",
     "SymplecticEulerGpu", "HybridMonteCarloDiagnostic", "MetropolisHastingsSweep",
+    "BatchedChains", "ChainSpec",
     "LatticeInitializationError", "StateInitializationError", "MultiIntegrationError",
     "LqcdError", "F32", "F64", "DRIFT_ADDITIVE", "DRIFT_EXP", "SCHEME_REFERENCE",
     "SCHEME_LEAPFROG",
@@ -531,3 +533,125 @@ class MetropolisHastingsSweep:
         self._prob_replace_mean = st.prob_replace_mean
         self._number_replace_last = int(st.number_replace_last)
         return state
+
+
+class ChainSpec:
+    """One chain of a batch: extents, beta, lattice spacing and seed."""
+
+    def __init__(self, extents: Sequence[int], beta: float, seed: int, size: float = 1.0):
+        self.extents = [int(x) for x in extents]
+        self.beta = float(beta)
+        self.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+        self.size = float(size)
+
+
+class BatchedChains:
+    """Independent HMC chains (each a ``LatticeStateDefault`` with its own
+    ``HybridMonteCarloDiagnostic``, hybrid_monte_carlo.rs:465-471, 584-613)
+    resident together and advanced by one launch per MD step; a trajectory is
+    replayed from a captured HIP graph.  All chains share D and precision.
+
+    Trajectory n of chain c draws momenta as ``lqcd_efield_random(seed_c, n,
+    0.5 / beta_c)`` and its accept word from the chain's Philox stream, so a
+    chain reproduces a single lattice driven with the same draws."""
+
+    def __init__(self, chains: Sequence[ChainSpec], d: int = 4, precision: int = F32,
+                 ctx: Optional[Context] = None):
+        self.ctx = ctx or Context.default()
+        n = len(chains)
+        specs = (F.ChainSpec * max(n, 1))()
+        for i, ch in enumerate(chains):
+            if len(ch.extents) != d:
+                raise LatticeInitializationError("every chain needs D extents")
+            for k, x in enumerate(ch.extents):
+                specs[i].extent[k] = x
+            specs[i].beta = ch.beta
+            specs[i].size = ch.size
+            specs[i].seed = ch.seed
+        h = C.c_void_p()
+        _raise(F.lib().lqcd_batch_create(self.ctx.handle, d, precision, specs, n, C.byref(h)))
+        self.handle = h
+        self.d = d
+        self.precision = precision
+        self.chains = list(chains)
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                F.lib().lqcd_batch_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+    def __len__(self) -> int:
+        return len(self.chains)
+
+    def number_of_links(self, chain: int) -> int:
+        nl = C.c_size_t(0)
+        _raise(F.lib().lqcd_batch_chain_info(self.handle, chain, None, C.byref(nl)))
+        return int(nl.value)
+
+    def total_sites(self) -> int:
+        ns = C.c_size_t(0)
+        _raise(F.lib().lqcd_batch_info(self.handle, None, C.byref(ns)))
+        return int(ns.value)
+
+    def cold_start(self) -> None:
+        _raise(F.lib().lqcd_batch_cold_start(self.handle))
+
+    def hot_start(self) -> None:
+        _raise(F.lib().lqcd_batch_hot_start(self.handle))
+
+    def link_matrix(self, chain: int) -> np.ndarray:
+        n = self.number_of_links(chain)
+        out = np.empty((n, 18))
+        _raise(F.lib().lqcd_batch_links_download(self.handle, chain, F.ptr(out), n))
+        return out
+
+    def set_link_matrix(self, chain: int, links) -> None:
+        a = np.ascontiguousarray(links, dtype=np.float64).reshape(-1, 18)
+        _raise(F.lib().lqcd_batch_links_upload(self.handle, chain, F.ptr(a), a.shape[0]))
+
+    def plaquette(self) -> np.ndarray:
+        out = np.empty(len(self.chains))
+        _raise(F.lib().lqcd_batch_plaquette(self.handle, F.ptr(out)))
+        return out
+
+    @property
+    def counter(self) -> int:
+        c = C.c_uint64(0)
+        _raise(F.lib().lqcd_batch_counter_get(self.handle, C.byref(c)))
+        return int(c.value)
+
+    @counter.setter
+    def counter(self, value: int) -> None:
+        _raise(F.lib().lqcd_batch_counter_set(self.handle, int(value)))
+
+    def hmc(self, delta_t: float, number_of_steps: int, n_trajectories: int = 1,
+            drift: int = DRIFT_EXP, scheme: int = SCHEME_LEAPFROG,
+            reunitarize: bool = False) -> np.ndarray:
+        """Runs the trajectories; returns stats ``(n_trajectories, nchain)``
+        with fields h_old, h_new, delta_h, probability, plaquette, accepted."""
+        stats = np.zeros((n_trajectories, len(self.chains)), dtype=F.CHAIN_STATS_DTYPE)
+        _raise(F.lib().lqcd_batch_hmc(self.handle, delta_t, number_of_steps, drift, scheme,
+                                      n_trajectories, int(bool(reunitarize)),
+                                      stats.ctypes.data_as(C.c_void_p)))
+        return stats
+
+    def autotune(self, drift: int = DRIFT_EXP) -> int:
+        c = C.c_int(0)
+        _raise(F.lib().lqcd_batch_autotune(self.handle, drift, C.byref(c)))
+        return int(c.value)
+
+    @property
+    def config(self) -> int:
+        c = C.c_int(0)
+        _raise(F.lib().lqcd_batch_config_get(self.handle, C.byref(c)))
+        return int(c.value)
+
+    @config.setter
+    def config(self, value: int) -> None:
+        _raise(F.lib().lqcd_batch_config_set(self.handle, int(value)))
+
+    def set_graph(self, enable: bool) -> None:
+        _raise(F.lib().lqcd_batch_graph_enable(self.handle, int(bool(enable))))

@@ -54,6 +54,35 @@ class MhStats(C.Structure):
     _fields_ = [("prob_replace_mean", C.c_double), ("number_replace_last", C.c_size_t)]
 
 
+class ChainSpec(C.Structure):
+    _fields_ = [
+        ("extent", C.c_size_t * 4),
+        ("beta", C.c_double),
+        ("size", C.c_double),
+        ("seed", C.c_uint64),
+    ]
+
+
+class ChainStats(C.Structure):
+    _fields_ = [
+        ("h_old", C.c_double),
+        ("h_new", C.c_double),
+        ("delta_h", C.c_double),
+        ("probability", C.c_double),
+        ("plaquette", C.c_double),
+        ("accepted", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+# numpy view of an lqcd_chain_stats array
+CHAIN_STATS_DTYPE = np.dtype([
+    ("h_old", np.float64), ("h_new", np.float64), ("delta_h", np.float64),
+    ("probability", np.float64), ("plaquette", np.float64),
+    ("accepted", np.int32), ("reserved", np.int32),
+])
+
+
 _P = C.c_void_p
 _D = C.c_double
 _S = C.c_size_t
@@ -98,6 +127,22 @@ SIGNATURES = {
     "lqcd_md_variant_get": (_I, [_P, C.POINTER(C.c_int)]),
     "lqcd_profile_enable": (_I, [_P, C.c_uint]),
     "lqcd_profile_read": (_I, [_P, _I, C.POINTER(C.c_size_t), C.POINTER(_D)]),
+    "lqcd_batch_create": (_I, [_P, _I, _I, C.POINTER(ChainSpec), _S, C.POINTER(_P)]),
+    "lqcd_batch_destroy": (_I, [_P]),
+    "lqcd_batch_info": (_I, [_P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "lqcd_batch_chain_info": (_I, [_P, _S, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "lqcd_batch_cold_start": (_I, [_P]),
+    "lqcd_batch_hot_start": (_I, [_P]),
+    "lqcd_batch_links_upload": (_I, [_P, _S, _P, _S]),
+    "lqcd_batch_links_download": (_I, [_P, _S, _P, _S]),
+    "lqcd_batch_plaquette": (_I, [_P, _P]),
+    "lqcd_batch_counter_get": (_I, [_P, C.POINTER(C.c_uint64)]),
+    "lqcd_batch_counter_set": (_I, [_P, _U64]),
+    "lqcd_batch_hmc": (_I, [_P, _D, _S, _I, _I, _S, _I, _P]),
+    "lqcd_batch_autotune": (_I, [_P, _I, C.POINTER(C.c_int)]),
+    "lqcd_batch_config_set": (_I, [_P, _I]),
+    "lqcd_batch_config_get": (_I, [_P, C.POINTER(C.c_int)]),
+    "lqcd_batch_graph_enable": (_I, [_P, _I]),
 }
 
 
