@@ -16,6 +16,19 @@ duration from HIP events recorded around every launch in the timed region.
 cpu_baseline = the oracle (C restatement of the reference's rayon
 SymplecticEulerRayon step, f64, OpenMP over all host threads) on a bounded
 sample of the same lattice.
+
+Other workloads (--workload), one JSON line each, same keys:
+  batched  64 independent chains, 16 each of 8^4, 12^4, 16^4, 20^4, beta
+           5.6..6.2, FP32 links with FP64 dH, hot start per chain seed, 20
+           leapfrog steps dt = 0.01 per trajectory, one launch per MD step,
+           trajectories replayed from a captured HIP graph, autotuned shape.
+  large    48^3 x 96 FP32-mixed HMC, beta = 6.2, cold start, trajectories of
+           50 leapfrog steps dt = 0.02, plaquette after every trajectory.
+  mh3d     D = 3 128^3 FP64 checkerboard Metropolis sweeps (spread 0.1,
+           beta = 6.0), proposals and accept draws generated on the device.
+  small    8^4 FP64: plaquette, HMC trajectories of 20 steps dt = 0.01
+           (reference kick-drift-kick scheme, additive drift) and Metropolis
+           sweeps (spread 0.1) with reunitarisation every 10 sweeps.
 """
 import argparse
 import json
@@ -23,6 +36,8 @@ import math
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -53,6 +68,10 @@ def parse():
     p.add_argument("--seed", type=int, default=0x457893F44AB067F0)
     p.add_argument("--copy-only", action="store_true",
                    help="only run the 1 GiB streaming copy (byte-counter calibration)")
+    p.add_argument("--workload", default="hmc32",
+                   choices=["hmc32", "batched", "large", "mh3d", "small"])
+    p.add_argument("--cpu-seconds", type=float, default=10.0,
+                   help="target length of the CPU baseline sample of the other workloads")
     return p.parse_args()
 
 
@@ -146,6 +165,275 @@ def cpu_baseline(args):
     }
 
 
+def contract_line(args, metric, value, unit, steps, seconds, dtype, config, roofline, cpu, **extra):
+    line = {
+        "metric": metric,
+        "value": round(value, 1),
+        "unit": unit,
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * seconds / steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic",
+        "config": config,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    return line
+
+
+def kernel_rate(ctx, kind, bytes_per_launch):
+    n, ms = ctx.profile_read(kind)
+    if n == 0:
+        return None
+    avg_s = ms / n / 1e3
+    gbs = bytes_per_launch / avg_s / 1e9
+    return {"launches": n, "avg_us": round(avg_s * 1e6, 2), "GB_s": round(gbs, 1),
+            "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_launch": int(bytes_per_launch)}
+
+
+def roofline_from(rate, kernel):
+    return {"bound": "hbm", "achieved": rate["GB_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": rate["hbm_frac"], "traffic": None, "kernel": kernel,
+            "bytes_per_launch": rate["bytes_per_launch"]}
+
+
+def cpu_md_sample(extents, beta, dt, seconds, seed):
+    """The oracle's kick-drift-kick f64 MD steps (SymplecticEulerRayon) on a
+    lattice of the given extents, as many steps as fit in about `seconds`."""
+    import oracle as O
+    ol = O.OracleLattice(len(extents), extents)
+    u = ol.hot_start(seed)
+    e = ol.random_efield(seed + 1, 0.5 / beta)
+    t0 = time.perf_counter()
+    u, e = ol.simulate_symplectic_n(u, e, dt, 1, O.DRIFT_ADDITIVE, O.SCHEME_REFERENCE)
+    one = time.perf_counter() - t0
+    n = max(1, int(seconds / max(one, 1e-6)) - 1)
+    t0 = time.perf_counter()
+    ol.simulate_symplectic_n(u, e, dt, n, O.DRIFT_ADDITIVE, O.SCHEME_REFERENCE)
+    t = time.perf_counter() - t0
+    threads = O.get_threads()
+    return {"value": ol.nlink * n / t, "unit": "link updates/s", "cores": threads, "kind": "port",
+            "sample": (f"{n} kick-drift-kick f64 MD steps (SymplecticEulerRayon restatement) of the "
+                       f"oracle on a {'x'.join(map(str, extents))} lattice, {threads} OpenMP threads, "
+                       f"{t:.1f} s")}
+
+
+def bench_batched(args, lqcd):
+    F = lqcd._ffi
+    sizes, n_steps, dt = [8, 12, 16, 20], 20, 0.01
+    specs = [lqcd.ChainSpec([sizes[k // 16]] * 4, 5.6 + 0.6 * k / 63, args.seed + k) for k in range(64)]
+    b = lqcd.BatchedChains(specs, 4, lqcd.F32)
+    ctx = b.ctx
+    b.hot_start()
+    config = b.autotune(lqcd.DRIFT_EXP)
+    nlink = sum(b.number_of_links(i) for i in range(len(specs)))
+    # eager pass with a timing event around every launch
+    b.set_graph(False)
+    b.hmc(dt, n_steps, 1)
+    ctx.profile_enable((1 << F.KERNEL_KICK_DRIFT) | (1 << F.KERNEL_KICK) | (1 << F.KERNEL_PLAQUETTE)
+                       | (1 << F.KERNEL_MOMENTA))
+    t0 = time.perf_counter()
+    b.hmc(dt, n_steps, 2)
+    eager_s = (time.perf_counter() - t0) / 2
+    kd = kernel_rate(ctx, F.KERNEL_KICK_DRIFT, nlink * REALS_KICK_DRIFT * 4)
+    kick = kernel_rate(ctx, F.KERNEL_KICK, nlink * REALS_KICK * 4)
+    sums = kernel_rate(ctx, F.KERNEL_PLAQUETTE, nlink * (REALS_PLAQ + REALS_EENERGY) * 4)
+    mom = kernel_rate(ctx, F.KERNEL_MOMENTA, nlink * REALS_MOMENTA * 4)
+    ctx.profile_enable(0)
+    # graph replays
+    b.set_graph(True)
+    b.hmc(dt, n_steps, max(args.warmup, 1))
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    stats = b.hmc(dt, n_steps, args.steps)
+    seconds = time.perf_counter() - t0
+    value = nlink * n_steps * args.steps / seconds
+    groups = {}
+    for g, L in enumerate(sizes):
+        s = stats[:, 16 * g:16 * (g + 1)]
+        groups[f"{L}^4"] = {"acceptance": round(float(s["accepted"].mean()), 4),
+                            "mean_abs_dH": float(abs(s["delta_h"]).mean()),
+                            "plaquette_last": float(s["plaquette"][-1].mean())}
+    shape = {0: "64", 1: "128", 2: "256"}[config % 3] + (" per-site" if (config // 3) % 2 else " per-link") \
+        + (" persistent" if config >= 6 else " full-grid")
+    cfg = {"workload": ("64 chains (16 each of 8^4, 12^4, 16^4, 20^4), beta 5.6..6.2, FP32 links/E with "
+                        "FP64 dH, hot start per chain seed, 20 leapfrog steps dt=0.01 per trajectory, "
+                        "exp link update, trajectories replayed from a captured HIP graph"),
+           "chains": 64, "sites": b.total_sites(), "links": nlink, "md_steps": n_steps, "dt": dt,
+           "launch_shape": {"config": config, "shape": shape}}
+    cpu = cpu_md_sample([16] * 4, 5.9, dt, args.cpu_seconds, args.seed) if args.cpu_md_steps > 0 else None
+    return contract_line(
+        args, "link updates/s, 64 batched chains", value, "link updates/s", args.steps, seconds, "f32",
+        cfg, roofline_from(kd, "k_batch_md<4,float,EXP> (fused force + kick + exp drift, all chains)"), cpu,
+        chain_traj_per_s=round(64 * args.steps / seconds, 2),
+        graph_traj_ms=round(1e3 * seconds / args.steps, 3), eager_traj_ms=round(1e3 * eager_s, 3),
+        kernels={"kick_drift": kd, "kick": kick, "sums": sums, "momenta": mom}, by_size=groups)
+
+
+def hmc_leg(lqcd, precision, extents, beta, dt, md_steps, steps, warmup, seed, cold=True):
+    F = lqcd._ffi
+    d = len(extents)
+    if cold:
+        st = lqcd.LatticeStateDefault.new_cold(1.0, beta, extents, d, precision)
+    else:
+        st = lqcd.LatticeStateDefault.new_determinist(1.0, beta, extents, seed, d, precision)
+    hmc = lqcd.HybridMonteCarloDiagnostic(dt, md_steps, lqcd.SymplecticEulerGpu(lqcd.DRIFT_EXP,
+                                                                               lqcd.SCHEME_LEAPFROG), seed)
+    ctx = st._dev.ctx
+    for _ in range(warmup):
+        hmc.next_element(st)
+        st.normalize_link_matrices()
+    ctx.profile_enable(1 << F.KERNEL_KICK_DRIFT)
+    ctx.synchronize()
+    plaq, dh, acc = [], [], 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        hmc.next_element(st)
+        st.normalize_link_matrices()
+        plaq.append(st.average_trace_plaquette().real / 3.0)
+        dh.append(hmc.last_stats.delta_h)
+        acc += hmc.last_stats.accepted
+    ctx.synchronize()
+    seconds = time.perf_counter() - t0
+    nlink = st.lattice().number_of_canonical_links_space()
+    elem = 4 if precision == lqcd.F32 else 8
+    kd = kernel_rate(ctx, F.KERNEL_KICK_DRIFT, nlink * REALS_KICK_DRIFT * elem)
+    ctx.profile_enable(0)
+    return {"seconds": seconds, "nlink": nlink, "plaquette": plaq, "dh": dh, "acceptance": acc / steps,
+            "kick_drift": kd, "md_variant": st.md_variant()}
+
+
+def bench_large(args, lqcd):
+    ext, beta, dt, md = [48, 48, 48, 96], 6.2, 0.02, 50
+    leg = hmc_leg(lqcd, lqcd.F32, ext, beta, dt, md, args.steps, args.warmup, args.seed)
+    value = leg["nlink"] * md * args.steps / leg["seconds"]
+    cfg = {"workload": ("HMC 48^3x96 D=4 beta=6.2, cold start, 50 leapfrog steps dt=0.02 per trajectory, "
+                        "exp link update, FP32 links/E with FP64 dH/plaquette/accept/reunitarisation, "
+                        "plaquette after every trajectory"),
+           "lattice": ext, "links": leg["nlink"], "md_steps": md, "dt": dt, "beta": beta,
+           "md_variant": leg["md_variant"]}
+    cpu = cpu_md_sample([24, 24, 24, 48], beta, dt, args.cpu_seconds, args.seed) if args.cpu_md_steps > 0 else None
+    return contract_line(
+        args, "link updates/s, 48^3x96 FP32-mixed HMC", value, "link updates/s", args.steps, leg["seconds"],
+        "f32", cfg, roofline_from(leg["kick_drift"], "MD kick+drift (fused force + kick + exp drift)"), cpu,
+        traj_per_s=round(args.steps / leg["seconds"], 3), acceptance=leg["acceptance"],
+        mean_abs_dH=float(sum(abs(x) for x in leg["dh"]) / len(leg["dh"])),
+        plaquette_per_trajectory=[round(p, 10) for p in leg["plaquette"]])
+
+
+def bench_mh3d(args, lqcd):
+    F = lqcd._ffi
+    L, beta, spread = 128, 6.0, 0.1
+    st = lqcd.LatticeStateDefault.new_determinist(1.0, beta, L, args.seed, 3, lqcd.F64)
+    mh = lqcd.MetropolisHastingsSweep(1, spread, args.seed)
+    ctx = st._dev.ctx
+    for _ in range(args.warmup):
+        mh.next_element(st)
+    ctx.profile_enable(1 << F.KERNEL_METROPOLIS)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        mh.next_element(st)
+    ctx.synchronize()
+    seconds = time.perf_counter() - t0
+    nlink = st.lattice().number_of_canonical_links_space()
+    # one class launch: every link read once (staples), its nlink/6 links written, p and flag stored
+    per_launch = nlink * 18 * 8 + nlink // 6 * (18 * 8 + 12)
+    rate = kernel_rate(ctx, F.KERNEL_METROPOLIS, per_launch)
+    ctx.profile_enable(0)
+    value = nlink * args.steps / seconds
+    cfg = {"workload": ("D=3 128^3 FP64 checkerboard Metropolis sweeps, beta=6.0, spread 0.1, 1 update per "
+                        "link, hot start, proposals and accept draws generated on the device"),
+           "lattice": [L] * 3, "links": nlink, "beta": beta, "spread": spread}
+    cpu = None
+    if args.cpu_md_steps > 0:
+        import oracle as O
+        ls = 32
+        ol = O.OracleLattice(3, ls)
+        u = ol.hot_start(args.seed)
+        rng = O.OracleRng(args.seed + 1)
+        n, t = 0, 0.0
+        while t < args.cpu_seconds:
+            t0 = time.perf_counter()
+            props = ol.mh_proposals(args.seed + 7 + n, spread, 1)
+            unis = np.array([rng.next_u64() for _ in range(ol.nlink)], dtype=np.uint64)
+            u, _ = ol.metropolis_sweep(u, beta, props, unis, O.ORDER_LEXICOGRAPHIC)
+            t += time.perf_counter() - t0
+            n += 1
+        cpu = {"value": ol.nlink * n / t, "unit": "link updates/s", "cores": 1, "kind": "port",
+               "sample": (f"{n} lexicographic single-threaded sweeps (MetropolisHastingsSweep restatement, "
+                          f"proposal and uniform draws included) of the oracle on 32^3, {t:.1f} s")}
+    return contract_line(
+        args, "link updates/s, D=3 128^3 FP64 Metropolis", value, "link updates/s", args.steps, seconds, "f64",
+        cfg, roofline_from(rate, "k_metropolis<3,double,RNG> (one checkerboard class)"), cpu,
+        sweeps_per_s=round(args.steps / seconds, 3), prob_replace_mean=mh.prob_replace_mean(),
+        number_replace_last=mh.number_replace_last())
+
+
+def bench_small(args, lqcd):
+    F = lqcd._ffi
+    L, beta, spread = 8, 6.0, 0.1
+    n_traj, md, dt, n_sweeps = 10, 20, 0.01, 100
+    st = lqcd.LatticeStateDefault.new_determinist(1.0, beta, L, args.seed, 4, lqcd.F64)
+    nlink = st.lattice().number_of_canonical_links_space()
+    hmc = lqcd.HybridMonteCarloDiagnostic(dt, md, lqcd.SymplecticEulerGpu(lqcd.DRIFT_ADDITIVE,
+                                                                        lqcd.SCHEME_REFERENCE), args.seed)
+    mh = lqcd.MetropolisHastingsSweep(1, spread, args.seed)
+    ctx = st._dev.ctx
+    hmc.next_element(st)
+    mh.next_element(st)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    plaq0 = st.average_trace_plaquette().real / 3.0
+    acc = 0
+    for _ in range(n_traj):
+        hmc.next_element(st)
+        acc += hmc.last_stats.accepted
+    t1 = time.perf_counter()
+    for k in range(n_sweeps):
+        mh.next_element(st)
+        if (k + 1) % 10 == 0:
+            st.normalize_link_matrices()
+    ctx.synchronize()
+    t2 = time.perf_counter()
+    updates = nlink * (n_traj * md + n_sweeps)
+    value = updates / (t2 - t0)
+    cfg = {"workload": ("8^4 FP64 beta=6.0 hot start: average_trace_plaquette, 10 HMC trajectories of 20 "
+                        "kick-drift-kick steps dt=0.01 (additive drift), 100 checkerboard Metropolis sweeps "
+                        "(spread 0.1) with normalize_link_matrices every 10 sweeps"),
+           "lattice": [L] * 4, "links": nlink}
+    cpu = None
+    if args.cpu_md_steps > 0:
+        import oracle as O
+        ol = O.OracleLattice(4, L)
+        u = ol.hot_start(args.seed)
+        rng = O.OracleRng(args.seed + 1)
+        c0 = time.perf_counter()
+        for k in range(n_traj):
+            e = ol.random_efield(args.seed + 100 + k, 0.5 / beta)
+            u, _ = ol.hmc_trajectory(u, e, beta, dt, md, rng.next_u64(), O.DRIFT_ADDITIVE, O.SCHEME_REFERENCE)
+        for k in range(n_sweeps):
+            props = ol.mh_proposals(args.seed + 7 + k, spread, 1)
+            unis = np.array([rng.next_u64() for _ in range(ol.nlink)], dtype=np.uint64)
+            u, _ = ol.metropolis_sweep(u, beta, props, unis, O.ORDER_LEXICOGRAPHIC)
+            if (k + 1) % 10 == 0:
+                u = ol.normalize_link_matrices(u)
+        c = time.perf_counter() - c0
+        cpu = {"value": updates / c, "unit": "link updates/s", "cores": O.get_threads(), "kind": "port",
+               "sample": (f"the whole workload on the oracle (MD over {O.get_threads()} OpenMP threads, "
+                          f"sweeps single-threaded, lexicographic), {c:.1f} s")}
+    return contract_line(
+        args, "link updates/s, 8^4 FP64 HMC + Metropolis", value, "link updates/s", 1, t2 - t0, "f64", cfg,
+        None, cpu, hmc_seconds=round(t1 - t0, 4), mh_seconds=round(t2 - t1, 4), hmc_acceptance=acc / n_traj,
+        plaquette_start=plaq0, plaquette_end=st.average_trace_plaquette().real / 3.0)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -157,6 +445,11 @@ def main():
     if args.copy_only:
         print(json.dumps({"stream_copy_GB_s": round(stream_copy_gbs(lqcd), 1),
                           "bytes": 1 << 30, "iters": 3}))
+        return
+    if args.workload != "hmc32":
+        run = {"batched": bench_batched, "large": bench_large, "mh3d": bench_mh3d,
+               "small": bench_small}[args.workload]
+        print(json.dumps(run(args, lqcd)))
         return
 
     mixed = run_leg(lqcd, lqcd.F32, args, args.steps, args.warmup, profile=True)

@@ -112,10 +112,14 @@ int capture_trajectory(lqcd_batch* b, double dt, size_t n, int drift, int scheme
     HIP_TRY(prepare_batch_md(b, b->config, drift));
     HIP_TRY(prepare_batch_md(b, b->config, BATCH_KICK));
     hipStream_t st = b->ctx->stream;
+    // timing events stay out of the graph: the graph holds kernel nodes only
+    const bool profiling = b->ctx->profiling;
+    b->ctx->profiling = false;
     HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     const hipError_t le = batch_trajectory(b, dt, n, drift, scheme, reunit);
     hipGraph_t graph = nullptr;
     const hipError_t ce = hipStreamEndCapture(st, &graph);
+    b->ctx->profiling = profiling;
     if (le != hipSuccess || ce != hipSuccess) {
         if (graph) (void)hipGraphDestroy(graph);
         (void)hipGetLastError();

@@ -312,7 +312,11 @@ static hipError_t batch_md_op(const lqcd_batch* b, int config, const void* U_in,
 hipError_t launch_batch_md(const lqcd_batch* b, int config, const void* U_in, void* U_out, double kick,
                            double dt, int op)
 {
-    return batch_md_op(b, config, U_in, U_out, kick, dt, op, true);
+    lqcd_prof_event ev;
+    prof_begin(b->ctx, op == BATCH_KICK ? PROF_KICK : PROF_KICK_DRIFT, &ev);
+    const hipError_t e = batch_md_op(b, config, U_in, U_out, kick, dt, op, true);
+    prof_end(b->ctx, &ev);
+    return e;
 }
 
 hipError_t prepare_batch_md(const lqcd_batch* b, int config, int op)
@@ -322,8 +326,11 @@ hipError_t prepare_batch_md(const lqcd_batch* b, int config, int op)
 
 hipError_t launch_batch_momenta(const lqcd_batch* b)
 {
+    lqcd_prof_event ev;
+    prof_begin(b->ctx, PROF_MOMENTA, &ev);
     BATCH_DISPATCH(b, k_batch_momenta<D, S><<<tile_grid(b), kBlock, 0, b->ctx->stream>>>(
                           (S*)b->E, b->d_chains, b->d_tile_chain, b->ntile, b->d_counter));
+    prof_end(b->ctx, &ev);
     return hipGetLastError();
 }
 
@@ -343,16 +350,22 @@ hipError_t launch_batch_cold(const lqcd_batch* b)
 
 hipError_t launch_batch_reunitarize(const lqcd_batch* b)
 {
+    lqcd_prof_event ev;
+    prof_begin(b->ctx, PROF_REUNIT, &ev);
     BATCH_DISPATCH(b, k_batch_links_map<D, S, 1><<<tile_grid(b), kBlock, 0, b->ctx->stream>>>(
                           (vec2_t<S>*)b->U[0], b->d_chains, b->d_tile_chain, b->ntile));
+    prof_end(b->ctx, &ev);
     return hipGetLastError();
 }
 
 hipError_t launch_batch_sums(const lqcd_batch* b, const void* U, const void* E, double* d_out)
 {
+    lqcd_prof_event ev;
+    prof_begin(b->ctx, PROF_PLAQUETTE, &ev);
     BATCH_DISPATCH(b, k_batch_tile_sums<D, S><<<tile_grid(b), kBlock, 0, b->ctx->stream>>>(
                           (const vec2_t<S>*)U, (const S*)E, b->d_chains, b->d_tile_chain, b->ntile,
                           b->d_partials));
+    prof_end(b->ctx, &ev);
     k_batch_chain_sums<<<(unsigned)b->chains.size(), kBlock, 0, b->ctx->stream>>>(b->d_partials, b->d_chains,
                                                                                  d_out);
     return hipGetLastError();
