@@ -45,7 +45,9 @@ typedef enum {
     LQCD_ERR_INVALID_ARGUMENT = 6,
     LQCD_ERR_UNSUPPORTED = 7,
     LQCD_ERR_DEVICE = 8,
-    LQCD_ERR_OUT_OF_MEMORY = 9
+    LQCD_ERR_OUT_OF_MEMORY = 9,
+    /* StateInitializationError::GaussProjectionError (src/error.rs:131-132) */
+    LQCD_ERR_GAUSS_PROJECTION = 10
 } lqcd_status;
 
 typedef enum { LQCD_F64 = 0, LQCD_F32 = 1 } lqcd_precision;
@@ -107,6 +109,21 @@ int lqcd_efield_cold(lqcd_lattice* lat);
  * sigma = 0.5 / beta, src/simulation/state.rs:1097).  Counter-based stream:
  * Philox4x32-10 keyed by seed, counter selects the draw. */
 int lqcd_efield_random(lqcd_lattice* lat, uint64_t seed, uint64_t counter, double sigma);
+
+/* EField::gauss (src/field.rs:1174-1195) at every site:
+ * G(x) = sum_i E_i(x) - U_{-i}(x) E_i(x - i) U_{-i}(x)^+, out [site][18]
+ * column-major 3x3 complex.  Computed in double arithmetic. */
+int lqcd_efield_gauss(lqcd_lattice* lat, double* out, size_t n_sites);
+/* EField::gauss_sum_div (src/field.rs:1199-1220): sum_x |Tr((sum_a T^a) G(x))| */
+int lqcd_efield_gauss_sum_div(lqcd_lattice* lat, double* out);
+/* EField::project_to_gauss (src/field.rs:1265-1337) applied to the lattice's
+ * E field: one project_to_gauss_step, then four more while gauss_sum_div >
+ * N * 4 * 8 * 10 * f64::EPSILON.  The iteration runs on double copies of E
+ * (also for f32 storage, where it is rounded once at the end).  The
+ * reference loops without bound; here at most max_iter checks are made
+ * (LQCD_ERR_GAUSS_PROJECTION if not converged or NaN, E unchanged then).
+ * iterations (may be NULL) gets the number of checks that failed. */
+int lqcd_efield_project_gauss(lqcd_lattice* lat, size_t max_iter, size_t* iterations);
 
 /* ---------------------------------------------------------- observables */
 /* LatticeState::average_trace_plaquette (src/simulation/state.rs:115-117,

@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <utility>
 
 #include "../../include/lqcd.h"
 #include "capi_util.hpp"
@@ -278,6 +279,11 @@ int lqcd_lattice_create(lqcd_ctx* ctx, int dim, const size_t* extent, double siz
     }
     L->g.nsite = n;
     L->g.npad = (n + 255) / 256 * 256;
+    // the device addresses the 9 planes of one direction with 32-bit offsets
+    if ((uint64_t)L->g.npad * 9 * 2 * L->elem > kMaxPlaneBytes) {
+        delete L;
+        return fail(LQCD_ERR_UNSUPPORTED, "lattice too large: the link planes of one direction exceed 4 GiB");
+    }
     L->nlink = (size_t)n * (size_t)dim;
     const size_t ubytes = (size_t)dim * 9 * (size_t)L->g.npad * 2 * L->elem;
     const size_t ebytes = (size_t)dim * 8 * (size_t)L->g.npad * L->elem;
@@ -314,6 +320,9 @@ int lqcd_lattice_destroy(lqcd_lattice* L)
     if (L->host_result) (void)hipHostFree(L->host_result);
     if (L->mh_p) (void)hipFree(L->mh_p);
     if (L->mh_acc) (void)hipFree(L->mh_acc);
+    if (L->gauss_g) (void)hipFree(L->gauss_g);
+    for (int k = 0; k < 2; ++k)
+        if (L->gauss_e[k]) (void)hipFree(L->gauss_e[k]);
     delete L;
     return LQCD_OK;
 }
@@ -415,6 +424,89 @@ int lqcd_efield_random(lqcd_lattice* L, uint64_t seed, uint64_t counter, double 
     CHECK_ARG(sigma >= 0.0 && std::isfinite(sigma), "sigma must be finite and >= 0");
     DeviceGuard dg(L->ctx->device);
     HIP_TRY(launch_device_momenta(L, L->E, seed, counter, sigma));
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    return LQCD_OK;
+}
+
+static int ensure_gauss_buffers(lqcd_lattice* L)
+{
+    const size_t npad = (size_t)L->g.npad;
+    if (!L->gauss_g) HIP_TRY(hipMalloc(&L->gauss_g, sizeof(double) * 2 * 9 * npad));
+    for (int k = 0; k < 2; ++k)
+        if (!L->gauss_e[k]) HIP_TRY(hipMalloc((void**)&L->gauss_e[k], sizeof(double) * 8 * npad * L->g.dim));
+    return LQCD_OK;
+}
+
+int lqcd_efield_gauss(lqcd_lattice* L, double* out, size_t n_sites)
+{
+    CHECK_ARG(L && out, "null argument");
+    if (n_sites != (size_t)L->g.nsite)
+        return fail(LQCD_ERR_INCOMPATIBLE_SIZE, "size of lattice and data are incompatible: expected " +
+                                                    std::to_string(L->g.nsite) + " sites, got " +
+                                                    std::to_string(n_sites));
+    DeviceGuard dg(L->ctx->device);
+    if (int rc = ensure_gauss_buffers(L)) return rc;
+    DevBuf st;
+    HIP_TRY(hipMalloc(&st.p, sizeof(double) * 18 * n_sites));
+    HIP_TRY(launch_efield_to_double(L, L->E, L->gauss_e[0]));
+    HIP_TRY(launch_gauss(L, L->gauss_e[0], L->result));
+    HIP_TRY(launch_gauss_to_aos(L, (double*)st.p));
+    HIP_TRY(hipMemcpyAsync(out, st.p, sizeof(double) * 18 * n_sites, hipMemcpyDeviceToHost, L->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    return LQCD_OK;
+}
+
+int lqcd_efield_gauss_sum_div(lqcd_lattice* L, double* out)
+{
+    CHECK_ARG(L && out, "null argument");
+    DeviceGuard dg(L->ctx->device);
+    if (int rc = ensure_gauss_buffers(L)) return rc;
+    HIP_TRY(launch_efield_to_double(L, L->E, L->gauss_e[0]));
+    HIP_TRY(launch_gauss(L, L->gauss_e[0], L->result));
+    if (int rc = read_results(L, 1)) return rc;
+    *out = L->host_result[0];
+    return LQCD_OK;
+}
+
+int lqcd_efield_project_gauss(lqcd_lattice* L, size_t max_iter, size_t* iterations)
+{
+    CHECK_ARG(L, "lattice is null");
+    DeviceGuard dg(L->ctx->device);
+    if (int rc = ensure_gauss_buffers(L)) return rc;
+    double* cur = L->gauss_e[0];
+    double* nxt = L->gauss_e[1];
+    // project_to_gauss, src/field.rs:1265-1294: one step, then while the
+    // deviation is above N * 4 * 8 * 10 * eps, four more steps
+    HIP_TRY(launch_efield_to_double(L, L->E, cur));
+    HIP_TRY(launch_gauss(L, cur, L->result));
+    HIP_TRY(launch_gauss_step(L, cur, nxt));
+    std::swap(cur, nxt);
+    const double threshold = 2.220446049250313e-16 * (double)(L->g.nsite * 4 * 8 * 10);
+    size_t it = 0;
+    bool converged = false;
+    for (; it < max_iter; ++it) {
+        HIP_TRY(launch_gauss(L, cur, L->result));
+        if (int rc = read_results(L, 1)) return rc;
+        const double div = L->host_result[0];
+        if (std::isnan(div)) {
+            if (iterations) *iterations = it;
+            return fail(LQCD_ERR_GAUSS_PROJECTION, "gauss projection could not finish: deviation is NaN");
+        }
+        if (div <= threshold) {
+            converged = true;
+            break;
+        }
+        for (int k = 0; k < 4; ++k) {
+            if (k > 0) HIP_TRY(launch_gauss(L, cur, L->result + 1));
+            HIP_TRY(launch_gauss_step(L, cur, nxt));
+            std::swap(cur, nxt);
+        }
+    }
+    if (iterations) *iterations = it;
+    if (!converged)
+        return fail(LQCD_ERR_GAUSS_PROJECTION, "gauss projection could not finish within " +
+                                                   std::to_string(max_iter) + " iterations");
+    HIP_TRY(launch_efield_from_double(L, cur, L->E));
     HIP_TRY(hipStreamSynchronize(L->ctx->stream));
     return LQCD_OK;
 }

@@ -246,6 +246,11 @@ int lqcd_batch_create(lqcd_ctx* ctx, int dim, int precision, const lqcd_chain_sp
         }
         d.g.nsite = n;
         d.g.npad = (n + 255) / 256 * 256;
+        if ((uint64_t)d.g.npad * 9 * 2 * b->elem > kMaxPlaneBytes) {
+            delete b;
+            return fail(LQCD_ERR_UNSUPPORTED,
+                        "chain too large: the link planes of one direction exceed 4 GiB");
+        }
         d.u_off = u_off;
         d.e_off = e_off;
         d.tile0 = tile0;

@@ -5,11 +5,21 @@
 // 16 B for f64).  E: plane (mu * 8 + a) holds e^a_mu(x) as a scalar S.  Every
 // plane has g.npad entries, so a wavefront over 64 consecutive sites issues
 // one fully coalesced 512 B (f32) / 1 KiB (f64) request per element.
+//
+// Accesses are raw buffer loads / stores: the resource covers the planes of
+// one direction (uniform, so it lives in scalar registers), the site enters
+// as a 32-bit byte offset and the element plane as the scalar offset.  No
+// per-lane 64-bit address arithmetic is left, and an access outside the
+// direction's planes reads zero instead of faulting.  The planes of one
+// direction must therefore span less than 4 GiB (kMaxPlaneBytes).
 #pragma once
 #include "geometry.hpp"
 #include "su3_device.hpp"
 
 namespace lqcd {
+
+// descriptor word 3 of a raw buffer resource on the gfx9 family
+constexpr int kBufferFlags = 0x00020000;
 
 template <typename S>
 struct vec2;
@@ -24,15 +34,54 @@ struct vec2<double> {
 template <typename S>
 using vec2_t = typename vec2<S>::type;
 
+// resource over `planes` planes of npad elements of type E starting at p
+template <typename E>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t planes_rsrc(const E* p, int64_t npad, int planes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)((uint64_t)planes * npad * sizeof(E)),
+                                             kBufferFlags);
+}
+
+template <typename E>
+__device__ __forceinline__ E buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff)
+{
+    if constexpr (sizeof(E) == 4) {
+        return __builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, (int)soff, 0));
+    } else if constexpr (sizeof(E) == 8) {
+        return __builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, (int)soff, 0));
+    } else {
+        static_assert(sizeof(E) == 16, "element of 4, 8 or 16 bytes");
+        return __builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, (int)soff, 0));
+    }
+}
+
+template <typename E>
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff, E v)
+{
+    if constexpr (sizeof(E) == 4) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)off, (int)soff, 0);
+    } else if constexpr (sizeof(E) == 8) {
+        using V = __attribute__((ext_vector_type(2))) uint32_t;
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, v), r, (int)off, (int)soff, 0);
+    } else {
+        static_assert(sizeof(E) == 16, "element of 4, 8 or 16 bytes");
+        using V = __attribute__((ext_vector_type(4))) uint32_t;
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V, v), r, (int)off, (int)soff, 0);
+    }
+}
+
 template <typename T, typename S>
 __device__ __forceinline__ mat3<T> load_link(const vec2_t<S>* __restrict__ U, int64_t npad, int mu,
                                              int64_t s)
 {
-    const vec2_t<S>* p = U + (int64_t)mu * 9 * npad + s;
+    using E = vec2_t<S>;
+    const __amdgpu_buffer_rsrc_t r = planes_rsrc(U + (int64_t)mu * 9 * npad, npad, 9);
+    const uint32_t off = (uint32_t)s * (uint32_t)sizeof(E);
+    const uint32_t plane = (uint32_t)npad * (uint32_t)sizeof(E);
     mat3<T> m;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-        const vec2_t<S> v = p[(int64_t)k * npad];
+        const E v = buf_load<E>(r, off, (uint32_t)k * plane);
         m.e[k] = {T(v.x), T(v.y)};
     }
     return m;
@@ -42,30 +91,37 @@ template <typename T, typename S>
 __device__ __forceinline__ void store_link(vec2_t<S>* __restrict__ U, int64_t npad, int mu, int64_t s,
                                            const mat3<T>& m)
 {
-    vec2_t<S>* p = U + (int64_t)mu * 9 * npad + s;
+    using E = vec2_t<S>;
+    const __amdgpu_buffer_rsrc_t r = planes_rsrc(U + (int64_t)mu * 9 * npad, npad, 9);
+    const uint32_t off = (uint32_t)s * (uint32_t)sizeof(E);
+    const uint32_t plane = (uint32_t)npad * (uint32_t)sizeof(E);
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-        vec2_t<S> v;
+        E v;
         v.x = S(m.e[k].re);
         v.y = S(m.e[k].im);
-        p[(int64_t)k * npad] = v;
+        buf_store<E>(r, off, (uint32_t)k * plane, v);
     }
 }
 
 template <typename T, typename S>
 __device__ __forceinline__ void load_e(const S* __restrict__ E, int64_t npad, int mu, int64_t s, T* e)
 {
-    const S* p = E + (int64_t)mu * 8 * npad + s;
+    const __amdgpu_buffer_rsrc_t r = planes_rsrc(E + (int64_t)mu * 8 * npad, npad, 8);
+    const uint32_t off = (uint32_t)s * (uint32_t)sizeof(S);
+    const uint32_t plane = (uint32_t)npad * (uint32_t)sizeof(S);
 #pragma unroll
-    for (int a = 0; a < 8; ++a) e[a] = T(p[(int64_t)a * npad]);
+    for (int a = 0; a < 8; ++a) e[a] = T(buf_load<S>(r, off, (uint32_t)a * plane));
 }
 
 template <typename T, typename S>
 __device__ __forceinline__ void store_e(S* __restrict__ E, int64_t npad, int mu, int64_t s, const T* e)
 {
-    S* p = E + (int64_t)mu * 8 * npad + s;
+    const __amdgpu_buffer_rsrc_t r = planes_rsrc(E + (int64_t)mu * 8 * npad, npad, 8);
+    const uint32_t off = (uint32_t)s * (uint32_t)sizeof(S);
+    const uint32_t plane = (uint32_t)npad * (uint32_t)sizeof(S);
 #pragma unroll
-    for (int a = 0; a < 8; ++a) p[(int64_t)a * npad] = S(e[a]);
+    for (int a = 0; a < 8; ++a) buf_store<S>(r, off, (uint32_t)a * plane, S(e[a]));
 }
 
 } // namespace lqcd

@@ -12,6 +12,9 @@
 namespace lqcd {
 
 constexpr int kMaxDim = 4;
+// the field planes of one direction (9 link elements) span less than 2^32
+// bytes: the device addresses them with 32-bit offsets
+constexpr uint64_t kMaxPlaneBytes = 0xFFFFFFFFull;
 
 struct Geom {
     int dim;
@@ -21,26 +24,28 @@ struct Geom {
     int64_t npad;    // row length of every SoA plane (>= nsite, multiple of 256)
 };
 
-// Offsets to the +dir and -dir neighbours of site s (periodic wrap).
+// Offsets to the +dir and -dir neighbours of site s (periodic wrap).  Site
+// indices of one lattice fit in 31 bits (the device field access uses 32-bit
+// byte offsets), so the offsets are 32-bit.
 template <int D>
 struct Neighbours {
-    int64_t up[D];   // s + up[d]   = site + d_hat
-    int64_t dn[D];   // s + dn[d]   = site - d_hat
+    int32_t up[D];   // s + up[d]   = site + d_hat
+    int32_t dn[D];   // s + dn[d]   = site - d_hat
 };
 
 template <int D>
 __device__ __forceinline__ Neighbours<D> neighbours(const Geom& g, int64_t s)
 {
     Neighbours<D> n;
-    int64_t r = s;
+    uint32_t r = (uint32_t)s;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        const int L = g.extent[d];
-        const int x = (d == D - 1) ? (int)r : (int)(r % L);
+        const uint32_t L = (uint32_t)g.extent[d];
+        const uint32_t x = (d == D - 1) ? r : r % L;
         r /= L;
-        const int64_t st = g.stride[d];
-        n.up[d] = (x == L - 1) ? -(int64_t)(L - 1) * st : st;
-        n.dn[d] = (x == 0) ? (int64_t)(L - 1) * st : -st;
+        const int32_t st = (int32_t)g.stride[d];
+        n.up[d] = (x == L - 1) ? -(int32_t)(L - 1) * st : st;
+        n.dn[d] = (x == 0) ? (int32_t)(L - 1) * st : -st;
     }
     return n;
 }
@@ -48,16 +53,16 @@ __device__ __forceinline__ Neighbours<D> neighbours(const Geom& g, int64_t s)
 template <int D>
 __device__ __forceinline__ int site_parity(const Geom& g, int64_t s)
 {
-    int64_t r = s;
-    int p = 0;
+    uint32_t r = (uint32_t)s;
+    uint32_t p = 0;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        const int L = g.extent[d];
-        const int x = (d == D - 1) ? (int)r : (int)(r % L);
+        const uint32_t L = (uint32_t)g.extent[d];
+        const uint32_t x = (d == D - 1) ? r : r % L;
         r /= L;
         p += x;
     }
-    return p & 1;
+    return (int)(p & 1);
 }
 
 } // namespace lqcd

@@ -42,6 +42,8 @@ struct lqcd_lattice {
     uint32_t* mh_acc = nullptr;    // per-link Metropolis accept flags
     int nblocks_reduce = 0;
     int md_variant = -1;           // work decomposition of the MD kernels (-1: not tuned)
+    void* gauss_g = nullptr;       // Gauss law G(x): 9 planes of double2
+    double* gauss_e[2] = {nullptr, nullptr};  // double copies of E for the projection
 };
 
 namespace lqcd {
@@ -110,6 +112,17 @@ hipError_t launch_metropolis_class(const lqcd_lattice* L, void* U, int mu, int p
 // the same class update with proposals and accept words drawn on the device
 hipError_t launch_metropolis_class_rng(const lqcd_lattice* L, void* U, int mu, int parity, double beta,
                                        size_t n_update, double spread, uint64_t seed, uint64_t counter);
+
+// ---- Gauss law (k_gauss.hip); links are U[cur], G goes to gauss_g ----
+// G(x) of the double E copy Ed, and div = sum_x |Tr(sum_a T^a G(x))| into d_div
+hipError_t launch_gauss(const lqcd_lattice* L, const double* Ed, double* d_div);
+// one project_to_gauss_step from Ed_in with the G of Ed_in (already in gauss_g)
+hipError_t launch_gauss_step(const lqcd_lattice* L, const double* Ed_in, double* Ed_out);
+hipError_t launch_gauss_to_aos(const lqcd_lattice* L, double* d_out);
+hipError_t launch_efield_to_double(const lqcd_lattice* L, const void* E, double* Ed);
+hipError_t launch_efield_from_double(const lqcd_lattice* L, const double* Ed, void* E);
+// sum of nblocks partials (slot 0) into d_out1
+hipError_t launch_final_reduce1(const lqcd_lattice* L, int nblocks, double* d_out1);
 
 // profiling hooks (capi.cpp)
 void prof_begin(lqcd_ctx* ctx, int kind, lqcd_prof_event* ev);

@@ -206,19 +206,62 @@ __device__ __constant__ const double kInvFactorial[26] = {
     6.446950284384474e-26,
 };
 
+// X^2 of a hermitian X (real diagonal a, b, c; X_01 = u, X_02 = v,
+// X_12 = w): hermitian, so three real diagonal and three complex upper
+// entries are computed and the lower ones conjugated.
+template <typename T>
+__device__ __forceinline__ void hermitian_square(const mat3<T>& X, mat3<T>& Y)
+{
+    const T a = X.e[0].re, b = X.e[4].re, c = X.e[8].re;
+    const cplx<T> u = X.e[1], v = X.e[2], w = X.e[5];
+    const T uu = fma(u.re, u.re, u.im * u.im);
+    const T vv = fma(v.re, v.re, v.im * v.im);
+    const T ww = fma(w.re, w.re, w.im * w.im);
+    Y.e[0] = {fma(a, a, uu + vv), T(0)};
+    Y.e[4] = {fma(b, b, uu + ww), T(0)};
+    Y.e[8] = {fma(c, c, vv + ww), T(0)};
+    const T ab = a + b, ac = a + c, bc = b + c;
+    // Y01 = (a + b) u + v conj(w)
+    Y.e[1] = {fma(ab, u.re, fma(v.re, w.re, v.im * w.im)), fma(ab, u.im, fma(v.im, w.re, -v.re * w.im))};
+    // Y02 = (a + c) v + u w
+    Y.e[2] = {fma(ac, v.re, fma(u.re, w.re, -u.im * w.im)), fma(ac, v.im, fma(u.re, w.im, u.im * w.re))};
+    // Y12 = (b + c) w + conj(u) v
+    Y.e[5] = {fma(bc, w.re, fma(u.re, v.re, u.im * v.im)), fma(bc, w.im, fma(u.re, v.im, -u.im * v.re))};
+    Y.e[3] = {Y.e[1].re, -Y.e[1].im};
+    Y.e[6] = {Y.e[2].re, -Y.e[2].im};
+    Y.e[7] = {Y.e[5].re, -Y.e[5].im};
+}
+
+// det X of a hermitian X (real):
+// abc - a|w|^2 - b|v|^2 - c|u|^2 + 2 Re(u w conj(v))
+template <typename T>
+__device__ __forceinline__ T hermitian_det(const mat3<T>& X)
+{
+    const T a = X.e[0].re, b = X.e[4].re, c = X.e[8].re;
+    const cplx<T> u = X.e[1], v = X.e[2], w = X.e[5];
+    const T uw_re = fma(u.re, w.re, -u.im * w.im);
+    const T uw_im = fma(u.re, w.im, u.im * w.re);
+    const T cyc = fma(uw_re, v.re, uw_im * v.im);
+    T r = a * fma(b, c, -fma(w.re, w.re, w.im * w.im));
+    r = fma(-b, fma(v.re, v.re, v.im * v.im), r);
+    r = fma(-c, fma(u.re, u.re, u.im * u.im), r);
+    return fma(T(2), cyc, r);
+}
+
 // Cayley-Hamilton recursion of su3_exp_i (src/su3.rs:832-855) started at
 // order K: q0 = 1/K!, then q0 <- 1/k! + d q2, q1 <- i (q0 - t q2),
 // q2 <- i q1 for k = K-1 .. 0 (the Taylor series to order K reduced by
-// X^3 = -t X - i d).  Fully unrolled.
+// X^3 = -t X - i d).  For hermitian X, d = i det X is imaginary: d = i delta.
+// Fully unrolled.
 template <int K, typename T>
-__device__ __forceinline__ void ch_recursion(cplx<T> d, T t, cplx<T>& q0, cplx<T>& q1, cplx<T>& q2)
+__device__ __forceinline__ void ch_recursion(T delta, T t, cplx<T>& q0, cplx<T>& q1, cplx<T>& q2)
 {
     q0 = {T(kInvFactorial[K]), T(0)};
     q1 = {T(0), T(0)};
     q2 = {T(0), T(0)};
 #pragma unroll
     for (int k = K - 1; k >= 0; --k) {
-        const cplx<T> dq2 = cmul(d, q2);
+        const cplx<T> dq2 = {-delta * q2.im, delta * q2.re};
         const cplx<T> q0n = {T(kInvFactorial[k]) + dq2.re, dq2.im};
         const cplx<T> w = {fma(-t, q2.re, q0.re), fma(-t, q2.im, q0.im)};
         const cplx<T> q1n = {-w.im, w.re};
@@ -259,12 +302,10 @@ __device__ __forceinline__ void exp_i(const T* v_in, mat3<T>& out)
     ss *= scale * scale;
     mat3<T> X, X2;
     adjoint_to_matrix(v, X);
-    mul<false, false>(X, X, X2);
+    hermitian_square(X, X2);
     const T t = T(-0.25) * ss;
-    const cplx<T> dx = det(X);
-    const cplx<T> d = {-dx.im, dx.re};
     cplx<T> q0, q1, q2;
-    ch_recursion<K>(d, t, q0, q1, q2);
+    ch_recursion<K>(hermitian_det(X), t, q0, q1, q2);
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         const cplx<T> x = X.e[k], y = X2.e[k];

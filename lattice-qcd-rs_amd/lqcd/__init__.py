@@ -59,7 +59,7 @@ def _raise(rc: int) -> None:
     msg = F.lib().lqcd_last_error().decode(errors="replace")
     if rc in (F.ERR_ZERO_DIMENSION, F.ERR_NON_POSITIVE_SIZE, F.ERR_DIM_TOO_SMALL):
         raise LatticeInitializationError(msg)
-    if rc == F.ERR_INCOMPATIBLE_SIZE:
+    if rc in (F.ERR_INCOMPATIBLE_SIZE, F.ERR_GAUSS_PROJECTION):
         raise StateInitializationError(msg)
     if rc == F.ERR_ZERO_INTEGRATION:
         raise MultiIntegrationError(msg)
@@ -288,14 +288,39 @@ class LatticeStateEFSyncDefault:
 
     @classmethod
     def new_random_e_state(cls, lattice_state: LatticeStateDefault, seed: int, counter: int = 0,
-                           sigma: Optional[float] = None):
-        """Momenta N(0, sigma) drawn on the device; sigma defaults to the
-        reference's 0.5 / beta (src/simulation/state.rs:1097).  The reference
-        then projects onto the Gauss-law surface; that projection is not
-        applied here."""
+                           sigma: Optional[float] = None, project: bool = True,
+                           max_iter: int = 1000):
+        """``new_random_e_state`` (src/simulation/state.rs:1093-1108): momenta
+        N(0, sigma) drawn on the device (sigma defaults to the reference's
+        0.5 / beta), then ``project_to_gauss`` unless ``project`` is False."""
         st = cls(lattice_state, None)
         st.reset_e_field(seed, counter, sigma)
+        if project:
+            st.project_to_gauss(max_iter)
         return st
+
+    def gauss(self) -> np.ndarray:
+        """``EField::gauss`` (src/field.rs:1174-1195) at every site, ``(nsite, 18)``
+        column-major 3x3 complex."""
+        n = self.lattice().number_of_points()
+        out = np.empty((n, 18))
+        _raise(F.lib().lqcd_efield_gauss(self.handle, F.ptr(out), n))
+        return out
+
+    def gauss_sum_div(self) -> float:
+        """``EField::gauss_sum_div`` (src/field.rs:1199-1220)."""
+        out = C.c_double()
+        _raise(F.lib().lqcd_efield_gauss_sum_div(self.handle, C.byref(out)))
+        return float(out.value)
+
+    def project_to_gauss(self, max_iter: int = 1000) -> int:
+        """``EField::project_to_gauss`` (src/field.rs:1265-1337) in place; the
+        reference's unbounded loop stops after ``max_iter`` checks with
+        ``StateInitializationError`` (GaussProjectionError).  Returns the
+        number of checks that failed before convergence."""
+        it = C.c_size_t(0)
+        _raise(F.lib().lqcd_efield_project_gauss(self.handle, max_iter, C.byref(it)))
+        return int(it.value)
 
     @property
     def handle(self):
@@ -416,16 +441,20 @@ def _splitmix64(x: int):
 class HybridMonteCarloDiagnostic:
     """``HybridMonteCarloDiagnostic`` (src/simulation/monte_carlo/hybrid_monte_carlo.rs:316-472).
 
-    ``next_element`` draws momenta N(0, 0.5/beta) on the device, integrates
-    ``number_of_steps`` steps of ``delta_t`` and accepts with
+    ``next_element`` draws momenta N(0, 0.5/beta) on the device, projects them
+    onto the Gauss-law surface as ``new_random_e_state`` does
+    (src/simulation/state.rs:1093-1108; ``project_gauss=False`` skips it),
+    integrates ``number_of_steps`` steps of ``delta_t`` and accepts with
     p = min(1, exp(H_old - H_new)).
     """
 
     def __init__(self, delta_t: float, number_of_steps: int, integrator: SymplecticEulerGpu,
-                 seed: int):
+                 seed: int, project_gauss: bool = True, gauss_max_iter: int = 1000):
         self._dt = float(delta_t)
         self._n = int(number_of_steps)
         self._integrator = integrator
+        self._project = bool(project_gauss)
+        self._gauss_max_iter = int(gauss_max_iter)
         self._seed = int(seed) & 0xFFFFFFFFFFFFFFFF
         self._rng_state = self._seed
         self._counter = 0
@@ -469,6 +498,8 @@ class HybridMonteCarloDiagnostic:
         sigma = 0.5 / state.beta()
         _raise(F.lib().lqcd_efield_random(state.handle, self._seed, self._counter, sigma))
         self._counter += 1
+        if self._project:
+            _raise(F.lib().lqcd_efield_project_gauss(state.handle, self._gauss_max_iter, None))
         self.trajectory(state, self._next_u64())
         return state
 

@@ -24,6 +24,7 @@ ERR_INVALID_ARGUMENT = 6
 ERR_UNSUPPORTED = 7
 ERR_DEVICE = 8
 ERR_OUT_OF_MEMORY = 9
+ERR_GAUSS_PROJECTION = 10
 
 F64, F32 = 0, 1
 DRIFT_ADDITIVE, DRIFT_EXP = 0, 1
@@ -108,6 +109,9 @@ SIGNATURES = {
     "lqcd_efield_download": (_I, [_P, _P, _S]),
     "lqcd_efield_cold": (_I, [_P]),
     "lqcd_efield_random": (_I, [_P, _U64, _U64, _D]),
+    "lqcd_efield_gauss": (_I, [_P, _P, _S]),
+    "lqcd_efield_gauss_sum_div": (_I, [_P, C.POINTER(_D)]),
+    "lqcd_efield_project_gauss": (_I, [_P, _S, C.POINTER(C.c_size_t)]),
     "lqcd_average_trace_plaquette": (_I, [_P, _P]),
     "lqcd_hamiltonian_links": (_I, [_P, _D, C.POINTER(_D)]),
     "lqcd_hamiltonian_efield": (_I, [_P, _D, C.POINTER(_D)]),

@@ -357,3 +357,54 @@ def test_wilson_adjoint_finite_differences_16x4(precision):
             assert abs((4 * f1 - f2) / 3 - pred) <= 1e-7 * abs(pred)
     print(f"adjoint: worst relative |fd - <Omega, grad>| over 64 directions = {worst:.2e}")
     assert worst <= 1e-5
+
+
+@pytest.mark.parametrize("dim,ext,cold", [(4, [4, 4, 4, 4], False), (3, [4, 6, 4], False),
+                                          (4, [4, 4, 2, 6], True)])
+def test_gauss_law_and_projection(dim, ext, cold):
+    # EField::gauss / gauss_sum_div / project_to_gauss (src/field.rs:1172-1337)
+    announce(f"gauss law D={dim} ext={ext} cold={cold}")
+    ol, u, st = make_state(dim, ext, cold=cold)
+    e = ol.random_efield(5, 0.5 / 6.0)
+    ef = lqcd.LatticeStateEFSyncDefault(st, e)
+    g = ef.gauss()
+    ref = np.array([ol.gauss(u, e, s) for s in range(ol.nsite)])
+    assert np.max(np.abs(g - ref)) <= 1e-13 * max(1.0, np.max(np.abs(ref)))
+    div_ref = ol.gauss_sum_div(u, e)
+    assert abs(ef.gauss_sum_div() - div_ref) <= 1e-12 * div_ref
+    it = ef.project_to_gauss(2000)
+    rc, e2, it_ref = ol.project_to_gauss(u, e, 2000)
+    assert rc == 0
+    print(f"projection: {it} checks on the device, {it_ref} in the oracle")
+    assert abs(it - it_ref) <= 1
+    assert np.max(np.abs(ef.e_field() - e2)) <= 1e-10
+    assert ef.gauss_sum_div() <= np.finfo(float).eps * ol.nsite * 320
+
+
+def test_gauss_projection_f32_storage_and_limits():
+    announce("gauss projection f32 / limits")
+    ol, u, _ = make_state(4, [4, 4, 4, 4])
+    e = ol.random_efield(9, 0.5 / 6.0)
+    rc, e2, _ = ol.project_to_gauss(u, e, 2000)
+    assert rc == 0
+    st32 = lqcd.LatticeStateDefault(lqcd.LatticeCyclic(1.0, 4, 4), 6.0, u, lqcd.F32)
+    ef32 = lqcd.LatticeStateEFSyncDefault(st32, e)
+    ef32.project_to_gauss(2000)
+    # projected in double from the f32 links, stored rounded to f32
+    assert np.max(np.abs(ef32.e_field() - e2)) <= 2e-6
+    # not converged within the limit: error, E left as it was
+    _, _, st = make_state(4, [4, 4, 4, 4])
+    ef = lqcd.LatticeStateEFSyncDefault(st, e)
+    with pytest.raises(lqcd.StateInitializationError):
+        ef.project_to_gauss(1)
+    assert np.array_equal(ef.e_field(), e)
+    bad = e.copy()
+    bad[3, 2] = np.nan
+    ef.set_e_field(bad)
+    with pytest.raises(lqcd.StateInitializationError):
+        ef.project_to_gauss(50)
+    # the HMC refresh projects by default (new_random_e_state)
+    hmc = lqcd.HybridMonteCarloDiagnostic(0.01, 3, lqcd.SymplecticEulerGpu(lqcd.DRIFT_EXP), 7)
+    hmc.next_element(st)
+    ef = lqcd.LatticeStateEFSyncDefault.new_random_e_state(st, 11, 0)
+    assert ef.gauss_sum_div() <= np.finfo(float).eps * ol.nsite * 320
