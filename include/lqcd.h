@@ -198,10 +198,30 @@ int lqcd_metropolis_sweep(lqcd_lattice* lat, double beta, size_t n_update,
  * built from 14 uniforms of Philox4x32-10 blocks 64 + 8j + (0..6), the
  * accept word from block 63; counter selects the sweep, key = seed.  The
  * stream differs from the reference's xoshiro draws but every draw has the
- * reference's distribution.  n_update <= 8000, 0 < spread < 1. */
+ * reference's distribution.  n_update <= 4000, 0 < spread < 1. */
 int lqcd_metropolis_sweep_random(lqcd_lattice* lat, double beta, size_t n_update,
                                  double spread, uint64_t seed, uint64_t counter,
                                  lqcd_mh_stats* stats);
+
+/* HeatBathSweep::next_element (src/simulation/monte_carlo/heat_bath.rs:73-123)
+ * in checkerboard order (as lqcd_metropolis_sweep), double arithmetic: per
+ * link, three SU(2) subgroup heat baths on the r, s, t sub-blocks of U A
+ * (su3.rs:428-660), each drawn from HeatBathDistribution(beta k)
+ * (statistics/distribution.rs:147-352, including the reference's
+ * sigma_3 = identity in complex_matrix_from_vec, so links leave SU(3) as in
+ * the reference) or su2::random_su2 when k is not normal.  Draws come from
+ * Philox4x32-10 (key seed, counter = sweep) instead of the rng argument;
+ * the norm sampler's rejection loop is cut after 64 tries per subgroup
+ * (factor = identity then), *exhausted (may be NULL) counts those.
+ * beta > 0, even extents. */
+int lqcd_heat_bath_sweep(lqcd_lattice* lat, double beta, uint64_t seed, uint64_t counter,
+                         size_t* exhausted);
+/* OverrelaxationSweepRotation::next_element
+ * (src/simulation/monte_carlo/overrelaxation.rs:66-91) in checkerboard order:
+ * new link = rot U^+ rot with rot = U V^+ of the SVD of staple^+, computed
+ * as the unitary polar factor (scaled Newton iteration, double arithmetic).
+ * Leaves the Wilson action unchanged.  Even extents. */
+int lqcd_overrelaxation_sweep(lqcd_lattice* lat);
 
 /* LatticeStateDefault::normalize_link_matrices (src/simulation/state.rs:754-756) */
 int lqcd_normalize_link_matrices(lqcd_lattice* lat);

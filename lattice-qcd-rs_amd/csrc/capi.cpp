@@ -708,6 +708,34 @@ int lqcd_metropolis_sweep(lqcd_lattice* L, double beta, size_t n_update, const d
     return metropolis_finish(L, stats);
 }
 
+int lqcd_heat_bath_sweep(lqcd_lattice* L, double beta, uint64_t seed, uint64_t counter, size_t* exhausted)
+{
+    CHECK_ARG(L, "lattice is null");
+    // HeatBathDistribution::new(beta * k) needs a positive finite parameter
+    CHECK_ARG(std::isfinite(beta) && beta > 0.0, "beta must be positive and finite");
+    DeviceGuard dg(L->ctx->device);
+    if (int rc = metropolis_prepare(L, beta, 1)) return rc;
+    for (int mu = 0; mu < L->g.dim; ++mu)
+        for (int par = 0; par < 2; ++par)
+            HIP_TRY(launch_heat_bath_class(L, L->U[L->cur], mu, par, beta, seed, counter));
+    HIP_TRY(launch_mh_stats(L, L->result));
+    if (int rc = read_results(L, 2)) return rc;
+    if (exhausted) *exhausted = (size_t)L->host_result[1];
+    return LQCD_OK;
+}
+
+int lqcd_overrelaxation_sweep(lqcd_lattice* L)
+{
+    CHECK_ARG(L, "lattice is null");
+    for (int i = 0; i < L->g.dim; ++i)
+        if (L->g.extent[i] % 2) return fail(LQCD_ERR_UNSUPPORTED, "checkerboard sweep needs even extents");
+    DeviceGuard dg(L->ctx->device);
+    for (int mu = 0; mu < L->g.dim; ++mu)
+        for (int par = 0; par < 2; ++par) HIP_TRY(launch_overrelax_class(L, L->U[L->cur], mu, par));
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    return LQCD_OK;
+}
+
 int lqcd_normalize_link_matrices(lqcd_lattice* L)
 {
     CHECK_ARG(L, "lattice is null");

@@ -51,8 +51,9 @@ namespace lqcd {
 constexpr int kBlock = 256;
 constexpr int kReduceSlots = 4;   // doubles per block partial
 constexpr int kMaxReduceBlocks = 4096;
-// device Metropolis proposals use Philox blocks 64 + 8 j + (0..6) < 2^16
-constexpr size_t kMaxMhUpdates = 8000;
+// device Metropolis proposals use Philox blocks 64 + 8 j + (0..6), below the
+// heat-bath blocks (32768..)
+constexpr size_t kMaxMhUpdates = 4000;
 
 enum ProfileKind {
     PROF_KICK_DRIFT = 0,
@@ -112,6 +113,12 @@ hipError_t launch_metropolis_class(const lqcd_lattice* L, void* U, int mu, int p
 // the same class update with proposals and accept words drawn on the device
 hipError_t launch_metropolis_class_rng(const lqcd_lattice* L, void* U, int mu, int parity, double beta,
                                        size_t n_update, double spread, uint64_t seed, uint64_t counter);
+
+// heat bath / overrelaxation of one (mu, parity) class (k_mc.hip); the heat
+// bath writes the per-link count of exhausted samplers to mh_acc (mh_p = 0)
+hipError_t launch_heat_bath_class(const lqcd_lattice* L, void* U, int mu, int parity, double beta, uint64_t seed,
+                                  uint64_t counter);
+hipError_t launch_overrelax_class(const lqcd_lattice* L, void* U, int mu, int parity);
 
 // ---- Gauss law (k_gauss.hip); links are U[cur], G goes to gauss_g ----
 // G(x) of the double E copy Ed, and div = sum_x |Tr(sum_a T^a G(x))| into d_div

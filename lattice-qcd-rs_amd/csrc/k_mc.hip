@@ -84,26 +84,13 @@ __device__ __forceinline__ uint64_t device_accept_word(uint64_t l, uint64_t seed
     return ((uint64_t)o.x << 32) | (uint64_t)o.y;
 }
 
-template <int D, typename S, bool RNG>
-__global__ __launch_bounds__(kBlock) void k_metropolis(vec2_t<S>* __restrict__ U, Geom g, int mu,
-                                                       int parity, double beta, int n_update,
-                                                       const double* __restrict__ proposals,
-                                                       const uint64_t* __restrict__ uniforms,
-                                                       double spread, uint64_t seed, uint64_t counter,
-                                                       double* __restrict__ p_out,
-                                                       uint32_t* __restrict__ acc_out)
+// staple A_mu(x) of monte_carlo/mod.rs:339-362 in double arithmetic:
+// sum_{nu != mu} U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+ + U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)
+template <int D, typename S>
+__device__ __forceinline__ void mc_staple(const vec2_t<S>* __restrict__ U, const Geom& g, int mu, int64_t s,
+                                          mat3<double>& a)
 {
-    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const int64_t half = g.nsite / 2;
-    if (k >= half) return;
-    // extent[0] is even: sites 2k and 2k+1 differ only in x_0 (even -> odd),
-    // so exactly one of them has the requested parity.
-    const int64_t s0 = 2 * k;
-    const int par0 = site_parity<D>(g, s0);
-    const int64_t s = s0 + ((par0 + parity) & 1);
     const Neighbours<D> nb = neighbours<D>(g, s);
-
-    mat3<double> a;
     set_zero(a);
     const int64_t s_pmu = s + nb.up[mu];
 #pragma unroll
@@ -127,6 +114,32 @@ __global__ __launch_bounds__(kBlock) void k_metropolis(vec2_t<S>* __restrict__ U
             mul_add<false, false>(xy, z, a);
         }
     }
+}
+
+// site of class (parity) handled by thread k: extent[0] is even, so sites 2k
+// and 2k+1 differ only in x_0 and exactly one of them has the parity
+template <int D>
+__device__ __forceinline__ int64_t class_site(const Geom& g, int64_t k, int parity)
+{
+    const int64_t s0 = 2 * k;
+    return s0 + ((site_parity<D>(g, s0) + parity) & 1);
+}
+
+template <int D, typename S, bool RNG>
+__global__ __launch_bounds__(kBlock) void k_metropolis(vec2_t<S>* __restrict__ U, Geom g, int mu,
+                                                       int parity, double beta, int n_update,
+                                                       const double* __restrict__ proposals,
+                                                       const uint64_t* __restrict__ uniforms,
+                                                       double spread, uint64_t seed, uint64_t counter,
+                                                       double* __restrict__ p_out,
+                                                       uint32_t* __restrict__ acc_out)
+{
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t half = g.nsite / 2;
+    if (k >= half) return;
+    const int64_t s = class_site<D>(g, k, parity);
+    mat3<double> a;
+    mc_staple<D, S>(U, g, mu, s, a);
     const mat3<double> u = load_link<double, S>(U, g.npad, mu, s);
     const int64_t li = s * D + mu;
     const int64_t nlink = g.nsite * D;
@@ -162,6 +175,254 @@ __global__ __launch_bounds__(kBlock) void k_metropolis(vec2_t<S>* __restrict__ U
     p_out[li] = p;
     acc_out[li] = accept ? 1u : 0u;
     if (accept) store_link<double, S>(U, g.npad, mu, s, nw);
+}
+
+// ---------------------------------------------------------------- heat bath
+// HeatBathSweep (simulation/monte_carlo/heat_bath.rs:73-123) with draws from
+// the device stream: subgroup g (0 r, 1 s, 2 t) of link l in sweep n uses
+// Philox blocks kHeatBathBlock + 256 g + k (key seed, counter n):
+//   k = 2t, 2t+1 (t < kHeatBathAttempts) attempt t of the norm sampler:
+//       w0 = r in [0,1), r_i = 1 - w_{i+1} in (0,1] for ModifiedNormal
+//   k = 128 + 2q, +1 (q < 2)  direction draw q: x_i = 2 w_i - 1
+//   k = 132 + 2q, +1 (q < 2)  random_su2 draw q: (re0, im0, re1, im1) = 2 w - 1
+constexpr int kHeatBathAttempts = 64;
+
+__device__ __forceinline__ void hb_draws(uint64_t l, uint32_t blk, uint64_t n, uint64_t seed, double (&w)[4])
+{
+#pragma unroll
+    for (uint32_t b = 0; b < 2; ++b) {
+        const uint4 o = philox_link(l, blk + b, n, seed);
+        w[2 * b] = u53(o.x, o.y);
+        w[2 * b + 1] = u53(o.z, o.w);
+    }
+}
+
+__device__ __forceinline__ cplx<double> conj2(cplx<double> a) { return {a.re, -a.im}; }
+
+// heat_bath_su2 (heat_bath.rs:73-87): k = sqrt(Re det M); if k is normal,
+// X v with v = M^+ / k and X from HeatBathDistribution(beta k)
+// (distribution.rs:193-214: x0 from HeatBathDistributionNorm, a random
+// direction scaled to sqrt(1 - x0^2), complex_matrix_from_vec with the
+// reference's sigma_3 = identity); otherwise su2::random_su2.  The norm
+// sampler gives up after kHeatBathAttempts tries: the factor is then the
+// identity and `exhausted` counts it.  2x2 matrices row-major.
+__device__ __forceinline__ void hb_su2(const cplx<double> (&st)[4], double beta, uint64_t l, int grp, uint64_t n,
+                                       uint64_t seed, cplx<double> (&out)[4], uint32_t& exhausted)
+{
+    const uint32_t base = kHeatBathBlock + 256u * (uint32_t)grp;
+    const cplx<double> det = {st[0].re * st[3].re - st[0].im * st[3].im - (st[1].re * st[2].re - st[1].im * st[2].im),
+                              st[0].re * st[3].im + st[0].im * st[3].re - (st[1].re * st[2].im + st[1].im * st[2].re)};
+    const double k = sqrt(det.re);
+    if (isfinite(k) && k >= 2.2250738585072014e-308) {
+        const double p = beta * k;
+        double x0 = 0.0;
+        bool ok = false;
+        for (int t = 0; t < kHeatBathAttempts && !ok; ++t) {
+            double w[4];
+            hb_draws(l, base + 2u * (uint32_t)t, n, seed, w);
+            const double r = w[0];
+            const double c = cos(2.0 * 3.14159265358979323846 * (1.0 - w[2]));
+            const double lambda = sqrt(-(log(1.0 - w[1]) + c * c * log(1.0 - w[3])) / (2.0 * p));
+            if (r * r <= 1.0 - lambda * lambda) {
+                x0 = 1.0 - 2.0 * lambda * lambda;
+                ok = true;
+            }
+        }
+        if (!ok) {
+            exhausted += 1;
+            out[0] = {1.0, 0.0};
+            out[1] = {0.0, 0.0};
+            out[2] = {0.0, 0.0};
+            out[3] = {1.0, 0.0};
+            return;
+        }
+        double x[3] = {0.0, 0.0, 0.0}, nrm = 0.0;
+        for (int q = 0; q < 2; ++q) {
+            double w[4];
+            hb_draws(l, base + 128u + 2u * (uint32_t)q, n, seed, w);
+            for (int i = 0; i < 3; ++i) x[i] = 2.0 * w[i] - 1.0;
+            nrm = sqrt((x[0] * x[0] + x[1] * x[1]) + x[2] * x[2]);
+            if (nrm > 2.220446049250313e-16) break;
+        }
+        const double sc = sqrt(1.0 - x0 * x0) / nrm;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) x[i] *= sc;
+        const cplx<double> X[4] = {{x0, x[2]}, {x[1], x[0]}, {-x[1], x[0]}, {x0, x[2]}};
+        const cplx<double> v[4] = {{st[0].re / k, -st[0].im / k}, {st[2].re / k, -st[2].im / k},
+                                   {st[1].re / k, -st[1].im / k}, {st[3].re / k, -st[3].im / k}};
+        out[0] = {X[0].re * v[0].re - X[0].im * v[0].im + X[1].re * v[2].re - X[1].im * v[2].im,
+                  X[0].re * v[0].im + X[0].im * v[0].re + X[1].re * v[2].im + X[1].im * v[2].re};
+        out[1] = {X[0].re * v[1].re - X[0].im * v[1].im + X[1].re * v[3].re - X[1].im * v[3].im,
+                  X[0].re * v[1].im + X[0].im * v[1].re + X[1].re * v[3].im + X[1].im * v[3].re};
+        out[2] = {X[2].re * v[0].re - X[2].im * v[0].im + X[3].re * v[2].re - X[3].im * v[2].im,
+                  X[2].re * v[0].im + X[2].im * v[0].re + X[3].re * v[2].im + X[3].im * v[2].re};
+        out[3] = {X[2].re * v[1].re - X[2].im * v[1].im + X[3].re * v[3].re - X[3].im * v[3].im,
+                  X[2].re * v[1].im + X[2].im * v[1].re + X[3].re * v[3].im + X[3].im * v[3].re};
+    } else {
+        double w[4] = {0.0, 0.0, 0.0, 0.0};
+        double nrm = 0.0;
+        for (int q = 0; q < 2; ++q) {
+            hb_draws(l, base + 132u + 2u * (uint32_t)q, n, seed, w);
+            for (int i = 0; i < 4; ++i) w[i] = 2.0 * w[i] - 1.0;
+            nrm = sqrt(((w[0] * w[0] + w[1] * w[1]) + w[2] * w[2]) + w[3] * w[3]);
+            if (isfinite(nrm) && nrm >= 2.2250738585072014e-308) break;
+        }
+        const cplx<double> a0 = {w[0] / nrm, w[1] / nrm}, b0 = {w[2] / nrm, w[3] / nrm};
+        out[0] = a0;
+        out[1] = b0;
+        out[2] = {-b0.re, b0.im};
+        out[3] = conj2(a0);
+    }
+}
+
+// sub-blocks of get_sub_block_r/s/t (su3.rs:559-640) and their embeddings
+// get_r/get_s/get_t (su3.rs:428-530): rows/cols (0,1), (0,2), (1,2)
+__device__ __forceinline__ void sub_block_rc(int grp, int& i, int& j)
+{
+    i = grp == 2 ? 1 : 0;
+    j = grp == 0 ? 1 : 2;
+}
+
+template <int D, typename S>
+__global__ __launch_bounds__(kBlock) void k_heat_bath(vec2_t<S>* __restrict__ U, Geom g, int mu, int parity,
+                                                      double beta, uint64_t seed, uint64_t counter,
+                                                      double* __restrict__ p_out, uint32_t* __restrict__ exhausted)
+{
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= g.nsite / 2) return;
+    const int64_t s = class_site<D>(g, k, parity);
+    mat3<double> a;
+    mc_staple<D, S>(U, g, mu, s, a);
+    const mat3<double> u = load_link<double, S>(U, g.npad, mu, s);
+    const int64_t li = s * D + mu;
+    mat3<double> ua, acc;
+    mul<false, false>(u, a, ua);
+    set_identity(acc);
+    uint32_t ex = 0;
+#pragma unroll
+    for (int grp = 0; grp < 3; ++grp) {
+        // project_to_su2_unorm of the sub-block of (acc U) A (su2.rs:155-157)
+        mat3<double> w;
+        mul<false, false>(acc, ua, w);
+        int i, j;
+        sub_block_rc(grp, i, j);
+        const cplx<double> m00 = w.e[i * 3 + i], m01 = w.e[i * 3 + j], m10 = w.e[j * 3 + i], m11 = w.e[j * 3 + j];
+        const cplx<double> tr = {m00.re + m11.re, -(m00.im + m11.im)};
+        const cplx<double> proj[4] = {{m00.re - m00.re + tr.re, m00.im + m00.im + tr.im},
+                                      {m01.re - m10.re, m01.im + m10.im},
+                                      {m10.re - m01.re, m10.im + m01.im},
+                                      {m11.re - m11.re + tr.re, m11.im + m11.im + tr.im}};
+        cplx<double> h[4];
+        hb_su2(proj, beta, (uint64_t)li, grp, counter, seed, h, ex);
+        mat3<double> f, t;
+        set_identity(f);
+        f.e[i * 3 + i] = h[0];
+        f.e[i * 3 + j] = h[1];
+        f.e[j * 3 + i] = h[2];
+        f.e[j * 3 + j] = h[3];
+        mul<false, false>(f, acc, t);
+        acc = t;
+    }
+    mat3<double> nw;
+    mul<false, false>(acc, u, nw);
+    store_link<double, S>(U, g.npad, mu, s, nw);
+    p_out[li] = 0.0;
+    exhausted[li] = ex;
+}
+
+// --------------------------------------------------------- overrelaxation
+// inverse of a 3x3 complex matrix by cofactors; returns det
+__device__ __forceinline__ cplx<double> inverse3(const mat3<double>& a, mat3<double>& inv)
+{
+    cplx<double> c[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            const cplx<double> p = cmul(a.e[i1 * 3 + j1], a.e[i2 * 3 + j2]);
+            const cplx<double> q = cmul(a.e[i1 * 3 + j2], a.e[i2 * 3 + j1]);
+            c[i * 3 + j] = {p.re - q.re, p.im - q.im};
+        }
+    cplx<double> det = {0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const cplx<double> t = cmul(a.e[j], c[j]);
+        det = {det.re + t.re, det.im + t.im};
+    }
+    const double dd = det.re * det.re + det.im * det.im;
+    const cplx<double> idet = {det.re / dd, -det.im / dd};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) inv.e[i * 3 + j] = cmul(c[j * 3 + i], idet);
+    return det;
+}
+
+// Unitary factor W of A = W P (W = U V^+ of the SVD A = U S V^+): scaled
+// Newton iteration X <- (g X + (g X)^{-+}) / 2 with g = |det X|^{-1/3}.
+__device__ __forceinline__ void polar_unitary(const mat3<double>& a, mat3<double>& x)
+{
+    x = a;
+    for (int it = 0; it < 100; ++it) {
+        mat3<double> inv;
+        const cplx<double> det = inverse3(x, inv);
+        const double gm = pow(sqrt(det.re * det.re + det.im * det.im), -1.0 / 3.0);
+        double diff = 0.0, nrm = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const cplx<double> t = {inv.e[j * 3 + i].re, -inv.e[j * 3 + i].im};
+                const cplx<double> v = {0.5 * (gm * x.e[i * 3 + j].re + t.re / gm),
+                                        0.5 * (gm * x.e[i * 3 + j].im + t.im / gm)};
+                diff += sqrt((v.re - x.e[i * 3 + j].re) * (v.re - x.e[i * 3 + j].re) +
+                             (v.im - x.e[i * 3 + j].im) * (v.im - x.e[i * 3 + j].im));
+                nrm += sqrt(v.re * v.re + v.im * v.im);
+                x.e[i * 3 + j] = v;
+            }
+        if (diff <= 1e-15 * nrm) break;
+    }
+}
+
+// OverrelaxationSweepRotation::get_modif (overrelaxation.rs:66-78):
+// rot = polar factor of staple^+, new link = rot U^+ rot
+template <int D, typename S>
+__global__ __launch_bounds__(kBlock) void k_overrelax(vec2_t<S>* __restrict__ U, Geom g, int mu, int parity)
+{
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= g.nsite / 2) return;
+    const int64_t s = class_site<D>(g, k, parity);
+    mat3<double> a, ad, rot, t, nw;
+    mc_staple<D, S>(U, g, mu, s, a);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) ad.e[i * 3 + j] = {a.e[j * 3 + i].re, -a.e[j * 3 + i].im};
+    polar_unitary(ad, rot);
+    const mat3<double> u = load_link<double, S>(U, g.npad, mu, s);
+    mul<false, true>(rot, u, t);
+    mul<false, false>(t, rot, nw);
+    store_link<double, S>(U, g.npad, mu, s, nw);
+}
+
+hipError_t launch_heat_bath_class(const lqcd_lattice* L, void* U, int mu, int parity, double beta, uint64_t seed,
+                                  uint64_t counter)
+{
+    const unsigned grid = grid_for(L->g.nsite / 2);
+    lqcd_prof_event ev;
+    prof_begin(L->ctx, PROF_METROPOLIS, &ev);
+    LQCD_DISPATCH_DS(L, k_heat_bath<D, S><<<grid, kBlock, 0, L->ctx->stream>>>(
+                            (vec2_t<S>*)U, L->g, mu, parity, beta, seed, counter, L->mh_p, L->mh_acc));
+    prof_end(L->ctx, &ev);
+    return hipGetLastError();
+}
+
+hipError_t launch_overrelax_class(const lqcd_lattice* L, void* U, int mu, int parity)
+{
+    const unsigned grid = grid_for(L->g.nsite / 2);
+    LQCD_DISPATCH_DS(L, k_overrelax<D, S><<<grid, kBlock, 0, L->ctx->stream>>>((vec2_t<S>*)U, L->g, mu, parity));
+    return hipGetLastError();
 }
 
 hipError_t launch_metropolis_class(const lqcd_lattice* L, void* U, int mu, int parity, double beta,

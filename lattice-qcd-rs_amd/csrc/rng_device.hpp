@@ -3,15 +3,20 @@
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).  Every draw of a link l is
 // addressed by a counter {lo32(l), hi16(l) | block << 16, lo32(n), hi32(n)}
 // with key = seed: n selects the refresh / sweep, block the use:
-//   0..3    momenta (one Box-Muller pair each)
-//   16..39  hot start (4 attempts x 6 blocks)
-//   63      Metropolis accept word
-//   64..    Metropolis proposals, 8 blocks per update
+//   0..3        momenta (one Box-Muller pair each)
+//   16..39      hot start (4 attempts x 6 blocks)
+//   62          accept word of a batched chain's trajectory (link 0)
+//   63          Metropolis accept word
+//   64..32767   Metropolis proposals, 8 blocks per update (at most 4000 updates)
+//   32768..     heat bath, 256 blocks per SU(2) subgroup
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace lqcd {
+
+// first Philox block of the heat-bath draws
+constexpr uint32_t kHeatBathBlock = 0x8000u;
 
 __device__ __forceinline__ uint4 philox4x32(uint4 c, uint2 k)
 {

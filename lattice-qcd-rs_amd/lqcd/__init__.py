@@ -13,6 +13,9 @@ reference (lattice-qcd-rs)              here
 ``SymplecticEulerRayon``                ``SymplecticEulerGpu``
 ``HybridMonteCarloDiagnostic``          ``HybridMonteCarloDiagnostic``
 ``MetropolisHastingsSweep``             ``MetropolisHastingsSweep`` (checkerboard)
+``HeatBathSweep``                       ``HeatBathSweep`` (checkerboard, device stream)
+``OverrelaxationSweepRotation``         ``OverrelaxationSweepRotation`` (checkerboard)
+``HybridMethodVec``                     ``HybridMethodVec``
 independent chains, one after another   ``BatchedChains`` (one launch per MD step)
 =====================================  ==========================================
 
@@ -34,7 +37,7 @@ from ._ffi import (DRIFT_ADDITIVE, DRIFT_EXP, F32, F64, SCHEME_LEAPFROG, SCHEME_
 __all__ = [
     "Context", "LatticeCyclic", "LatticeStateDefault", "LatticeStateEFSyncDefault",
     "SymplecticEulerGpu", "HybridMonteCarloDiagnostic", "MetropolisHastingsSweep",
-    "BatchedChains", "ChainSpec",
+    "BatchedChains", "ChainSpec", "HeatBathSweep", "OverrelaxationSweepRotation", "HybridMethodVec",
     "LatticeInitializationError", "StateInitializationError", "MultiIntegrationError",
     "LqcdError", "F32", "F64", "DRIFT_ADDITIVE", "DRIFT_EXP", "SCHEME_REFERENCE",
     "SCHEME_LEAPFROG",
@@ -686,3 +689,61 @@ class BatchedChains:
 
     def set_graph(self, enable: bool) -> None:
         _raise(F.lib().lqcd_batch_graph_enable(self.handle, int(bool(enable))))
+
+
+class HeatBathSweep:
+    """``HeatBathSweep`` (src/simulation/monte_carlo/heat_bath.rs:20-123) on the
+    device, checkerboard order.  The reference's ``rng`` is a Philox stream
+    keyed by ``seed``, one counter per sweep.  As in the reference, the SU(2)
+    factors use sigma_3 = identity, so links leave SU(3): follow with
+    ``normalize_link_matrices``."""
+
+    def __init__(self, seed: int = 0):
+        self._seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+        self._counter = 0
+        self._exhausted_last = 0
+
+    def counter(self) -> int:
+        return self._counter
+
+    def exhausted_last(self) -> int:
+        """Norm samplers of the last sweep that ran out of attempts."""
+        return self._exhausted_last
+
+    def next_element(self, state: LatticeStateDefault) -> LatticeStateDefault:
+        ex = C.c_size_t(0)
+        _raise(F.lib().lqcd_heat_bath_sweep(state.handle, state.beta(), self._seed, self._counter,
+                                            C.byref(ex)))
+        self._counter += 1
+        self._exhausted_last = int(ex.value)
+        return state
+
+
+class OverrelaxationSweepRotation:
+    """``OverrelaxationSweepRotation`` (src/simulation/monte_carlo/overrelaxation.rs:43-106)
+    on the device, checkerboard order; the Wilson action is unchanged."""
+
+    def next_element(self, state: LatticeStateDefault) -> LatticeStateDefault:
+        _raise(F.lib().lqcd_overrelaxation_sweep(state.handle))
+        return state
+
+
+class HybridMethodVec:
+    """``HybridMethodVec`` (src/simulation/monte_carlo/hybrid.rs:156-268): the
+    methods applied in order, each through ``monte_carlo_step``."""
+
+    def __init__(self, methods=None):
+        self._methods = list(methods or [])
+
+    def push_method(self, method) -> None:
+        self._methods.append(method)
+
+    def methods(self):
+        return list(self._methods)
+
+    def next_element(self, state):
+        if not self._methods:
+            raise ValueError("HybridMethodVecError::NoMethod")
+        for m in self._methods:
+            state = state.monte_carlo_step(m)
+        return state

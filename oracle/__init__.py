@@ -120,6 +120,11 @@ def _declare(L):
         "or_device_momenta": (None, [LP, _U64, _U64, _D, _P]),
         "or_device_hot_start": (None, [LP, _U64, _P]),
         "or_device_mh_inputs": (None, [LP, _U64, _U64, _D, _S, _P, _P]),
+        "or_su2_project_unorm": (None, [_P, _P]),
+        "or_heat_bath_sweep": (_I, [LP, _P, _D, _U64, _U64, _I, C.POINTER(C.c_size_t)]),
+        "or_heat_bath_x0": (_D, [_D, _U64, _U64, _U64]),
+        "or_polar_unitary": (None, [_P, _P]),
+        "or_overrelaxation_sweep": (_I, [LP, _P, _I]),
         "or_bernoulli_accept": (_I, [_D, _U64]),
         "or_clamp_probability": (_D, [_D]),
         "or_link_matrix": (None, [LP, _P, _S, _I, _I, _P]),
@@ -355,6 +360,22 @@ class OracleLattice:
             raise ValueError(f"or_metropolis_sweep failed: {rc}")
         return u, st
 
+    def heat_bath_sweep(self, u, beta, seed, counter, order=ORDER_CHECKERBOARD):
+        """The device heat-bath stream's sweep; returns (links, exhausted samplers)."""
+        u = np.array(u, copy=True)
+        ex = C.c_size_t(0)
+        rc = lib().or_heat_bath_sweep(self.ref, _ptr(u), beta, seed, counter, order, C.byref(ex))
+        if rc != 0:
+            raise ValueError(f"or_heat_bath_sweep failed: {rc}")
+        return u, int(ex.value)
+
+    def overrelaxation_sweep(self, u, order=ORDER_CHECKERBOARD):
+        u = np.array(u, copy=True)
+        rc = lib().or_overrelaxation_sweep(self.ref, _ptr(u), order)
+        if rc != 0:
+            raise ValueError(f"or_overrelaxation_sweep failed: {rc}")
+        return u
+
     def normalize_link_matrices(self, u):
         u = np.array(u, copy=True)
         lib().or_normalize_link_matrices(self.ref, _ptr(u))
@@ -432,6 +453,26 @@ def clamp_probability(x: float) -> float:
 
 def factorial_size_for_exp() -> int:
     return int(lib().or_factorial_size_for_exp())
+
+
+def su2_project_unorm(m2) -> np.ndarray:
+    """su2::project_to_su2_unorm of a 2x2 complex matrix."""
+    a = np.ascontiguousarray(np.asarray(m2, dtype=np.complex128).reshape(4))
+    v = np.ascontiguousarray(np.stack([a.real, a.imag], axis=1).reshape(8))
+    out = np.zeros(8)
+    lib().or_su2_project_unorm(_ptr(v), _ptr(out))
+    return (out[0::2] + 1j * out[1::2]).reshape(2, 2)
+
+
+def heat_bath_x0(p: float, l: int, n: int, seed: int) -> float:
+    return float(lib().or_heat_bath_x0(p, l, n, seed))
+
+
+def polar_unitary(m18) -> np.ndarray:
+    m = np.ascontiguousarray(m18, dtype=np.float64)
+    out = np.zeros(18)
+    lib().or_polar_unitary(_ptr(m), _ptr(out))
+    return out
 
 
 def philox4x32(ctr, key):

@@ -1217,6 +1217,282 @@ int or_metropolis_sweep(const or_lattice* l, double* links, double beta, size_t 
     return 0;
 }
 
+/* ------------------------------------------------------------------------ */
+/* heat bath: simulation/monte_carlo/heat_bath.rs:73-123,                    */
+/* statistics/distribution.rs:147-352, su2.rs:155-157,200-217, su3.rs:428-660 */
+/* ------------------------------------------------------------------------ */
+
+/* 2x2 matrices row-major [m00, m01, m10, m11] */
+static void m2_mul(const or_cplx a[4], const or_cplx b[4], or_cplx c[4])
+{
+    or_cplx r[4];
+    r[0] = cadd(cmul(a[0], b[0]), cmul(a[1], b[2]));
+    r[1] = cadd(cmul(a[0], b[1]), cmul(a[1], b[3]));
+    r[2] = cadd(cmul(a[2], b[0]), cmul(a[3], b[2]));
+    r[3] = cadd(cmul(a[2], b[1]), cmul(a[3], b[3]));
+    memcpy(c, r, sizeof(r));
+}
+
+/* su2::project_to_su2_unorm, src/su2.rs:155-157: m - m^+ + I conj(Tr m) */
+void or_su2_project_unorm(const or_cplx m[4], or_cplx out[4])
+{
+    or_cplx tr = cconj(cadd(m[0], m[3]));
+    or_cplx r[4];
+    r[0] = cadd(csub(m[0], cconj(m[0])), tr);
+    r[1] = csub(m[1], cconj(m[2]));
+    r[2] = csub(m[2], cconj(m[1]));
+    r[3] = cadd(csub(m[3], cconj(m[3])), tr);
+    memcpy(out, r, sizeof(r));
+}
+
+/* get_sub_block_r/s/t (src/su3.rs:559-640): block 0 = rows/cols (0,1),
+ * 1 = (0,2), 2 = (1,2) */
+static const int SUB_I[3] = {0, 0, 1};
+static const int SUB_J[3] = {1, 2, 2};
+static void sub_block(const or_mat3* m, int b, or_cplx out[4])
+{
+    int i = SUB_I[b], j = SUB_J[b];
+    out[0] = M(m, i, i); out[1] = M(m, i, j); out[2] = M(m, j, i); out[3] = M(m, j, j);
+}
+/* get_r/get_s/get_t (src/su3.rs:428-530): embed at the block, 1 elsewhere */
+static void embed_block(const or_cplx m[4], int b, or_mat3* out)
+{
+    int i = SUB_I[b], j = SUB_J[b];
+    mident(out);
+    M(out, i, i) = m[0]; M(out, i, j) = m[1]; M(out, j, i) = m[2]; M(out, j, j) = m[3];
+}
+
+/* Device heat-bath stream of link l in sweep n (key seed): subgroup g
+ * (0 r, 1 s, 2 t) uses Philox blocks HB + 256 g + k, HB = 0x8000:
+ *   k = 2t, 2t+1 (t < 64)  attempt t of the norm sampler: w0 = r in [0,1),
+ *                          r_i = 1 - w_{i+1} in (0,1] for ModifiedNormal
+ *   k = 128 + 2q, +1 (q < 2) direction draw q: x_i = 2 w_i - 1 (i < 3)
+ *   k = 132 + 2q, +1 (q < 2) random_su2 draw q: (re0, im0, re1, im1) = 2 w - 1 */
+#define OR_HB_BASE 0x8000u
+#define OR_HB_ATTEMPTS 64
+static void hb_draws(uint64_t l, uint32_t blk, uint64_t n, uint64_t seed, double w[4])
+{
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    for (int b = 0; b < 2; ++b) {
+        uint32_t ctr[4] = {(uint32_t)l, (uint32_t)((l >> 32) & 0xFFFFu) | ((blk + (uint32_t)b) << 16),
+                           (uint32_t)n, (uint32_t)(n >> 32)};
+        uint32_t o[4];
+        or_philox4x32(ctr, key, o);
+        w[2 * b] = u53(o[0], o[1]);
+        w[2 * b + 1] = u53(o[2], o[3]);
+    }
+}
+
+/* HeatBathSweep::heat_bath_su2 (heat_bath.rs:73-87) with the draws above.
+ * HeatBathDistributionNorm (distribution.rs:336-350): loop { r; lambda =
+ * ModifiedNormal (distribution.rs:~66-76); if r^2 <= 1 - lambda^2 return
+ * 1 - 2 lambda^2 }, here at most OR_HB_ATTEMPTS tries (then the subgroup
+ * update is the identity and *exhausted is incremented).  Returns the 2x2
+ * factor X v. */
+static void hb_su2(const or_cplx st[4], double beta, uint64_t l, int g, uint64_t n, uint64_t seed,
+                   or_cplx out[4], size_t* exhausted)
+{
+    const uint32_t base = OR_HB_BASE + 256u * (uint32_t)g;
+    /* staple.determinant().real().sqrt() */
+    or_cplx det = csub(cmul(st[0], st[3]), cmul(st[1], st[2]));
+    double k = sqrt(det.re);
+    if (isnormal(k)) {
+        const double p = beta * k;
+        double x0 = 0.0;
+        int ok = 0;
+        for (int t = 0; t < OR_HB_ATTEMPTS && !ok; ++t) {
+            double w[4];
+            hb_draws(l, base + 2u * (uint32_t)t, n, seed, w);
+            double r = w[0];
+            double r0 = 1.0 - w[1], r1 = 1.0 - w[2], r2 = 1.0 - w[3];
+            double c = cos(2.0 * M_PI * r1);
+            double lambda = sqrt(-(log(r0) + c * c * log(r2)) / (2.0 * p));
+            if (r * r <= 1.0 - lambda * lambda) {
+                x0 = 1.0 - 2.0 * lambda * lambda;
+                ok = 1;
+            }
+        }
+        if (!ok) {
+            *exhausted += 1;
+            out[0] = C_ONE; out[1] = C_ZERO; out[2] = C_ZERO; out[3] = C_ONE;
+            return;
+        }
+        double x[3] = {0.0, 0.0, 0.0}, nrm = 0.0;
+        for (int q = 0; q < 2; ++q) {
+            double w[4];
+            hb_draws(l, base + 128u + 2u * (uint32_t)q, n, seed, w);
+            for (int i = 0; i < 3; ++i) x[i] = 2.0 * w[i] - 1.0;
+            nrm = sqrt((x[0] * x[0] + x[1] * x[1]) + x[2] * x[2]);
+            if (nrm > F64_EPS) break;
+        }
+        double sc = sqrt(1.0 - x0 * x0) / nrm;
+        for (int i = 0; i < 3; ++i) x[i] *= sc;
+        /* complex_matrix_from_vec (su2.rs:134-140), PAULI_3 = identity quirk */
+        or_cplx X[4] = {cx(x0, x[2]), cx(x[1], x[0]), cx(-x[1], x[0]), cx(x0, x[2])};
+        /* v = staple^+ / k */
+        or_cplx v[4] = {cx(st[0].re / k, -st[0].im / k), cx(st[2].re / k, -st[2].im / k),
+                        cx(st[1].re / k, -st[1].im / k), cx(st[3].re / k, -st[3].im / k)};
+        m2_mul(X, v, out);
+    } else {
+        /* su2::random_su2 (su2.rs:200-217) */
+        double w[4];
+        double nrm = 0.0;
+        for (int q = 0; q < 2; ++q) {
+            hb_draws(l, base + 132u + 2u * (uint32_t)q, n, seed, w);
+            for (int i = 0; i < 4; ++i) w[i] = 2.0 * w[i] - 1.0;
+            nrm = sqrt(((w[0] * w[0] + w[1] * w[1]) + w[2] * w[2]) + w[3] * w[3]);
+            if (isnormal(nrm)) break;
+        }
+        or_cplx a = cx(w[0] / nrm, w[1] / nrm), b = cx(w[2] / nrm, w[3] / nrm);
+        out[0] = a; out[1] = b; out[2] = cx(-b.re, b.im); out[3] = cconj(a);
+    }
+}
+
+/* HeatBathSweep::get_modif (heat_bath.rs:90-110): three subgroup heat baths
+ * on U A, r U A, s r U A; the new link is t s r U. */
+static void hb_update_link(const or_lattice* l, double* links, size_t site, int dir, double beta,
+                           uint64_t seed, uint64_t n, size_t* exhausted)
+{
+    const size_t li = site * (size_t)l->dim + (size_t)dir;
+    or_mat3 u, a, w, f[3], acc, tmp;
+    load_mat(links + 18 * li, &u);
+    or_staple(l, links, site, dir, &a);
+    mident(&acc); /* product of the subgroup factors applied so far */
+    for (int g = 0; g < 3; ++g) {
+        or_mat3 left;
+        or_matmul(&acc, &u, &left);
+        or_matmul(&left, &a, &w);
+        or_cplx blk[4], proj[4], hb[4];
+        sub_block(&w, g, blk);
+        or_su2_project_unorm(blk, proj);
+        hb_su2(proj, beta, (uint64_t)li, g, n, seed, hb, exhausted);
+        embed_block(hb, g, &f[g]);
+        or_matmul(&f[g], &acc, &tmp);
+        acc = tmp;
+    }
+    or_matmul(&acc, &u, &tmp);
+    store_mat(links + 18 * li, &tmp);
+}
+
+/* HeatBathSweep::next_element_default (heat_bath.rs:112-123) with the device
+ * stream, checkerboard (or lexicographic) order as or_metropolis_sweep.
+ * Returns -2 for odd extents in checkerboard order. */
+int or_heat_bath_sweep(const or_lattice* l, double* links, double beta, uint64_t seed,
+                       uint64_t counter, int order, size_t* exhausted)
+{
+    size_t ex = 0;
+    if (order == OR_ORDER_LEXICOGRAPHIC) {
+        for (size_t s = 0; s < l->nsite; ++s)
+            for (int d = 0; d < l->dim; ++d) hb_update_link(l, links, s, d, beta, seed, counter, &ex);
+    } else {
+        for (int i = 0; i < l->dim; ++i)
+            if (l->extent[i] % 2) return -2;
+        for (int d = 0; d < l->dim; ++d)
+            for (int par = 0; par < 2; ++par)
+                for (size_t s = 0; s < l->nsite; ++s)
+                    if (or_site_parity(l, s) == par) hb_update_link(l, links, s, d, beta, seed, counter, &ex);
+    }
+    if (exhausted) *exhausted = ex;
+    return 0;
+}
+
+/* one draw of the heat-bath norm sampler (for the distribution tests):
+ * x0 of attempt sequence (l, g = 0, n, seed) with parameter p, or NaN when
+ * the attempts run out */
+double or_heat_bath_x0(double p, uint64_t l, uint64_t n, uint64_t seed)
+{
+    for (int t = 0; t < OR_HB_ATTEMPTS; ++t) {
+        double w[4];
+        hb_draws(l, OR_HB_BASE + 2u * (uint32_t)t, n, seed, w);
+        double r = w[0];
+        double r0 = 1.0 - w[1], r1 = 1.0 - w[2], r2 = 1.0 - w[3];
+        double c = cos(2.0 * M_PI * r1);
+        double lambda = sqrt(-(log(r0) + c * c * log(r2)) / (2.0 * p));
+        if (r * r <= 1.0 - lambda * lambda) return 1.0 - 2.0 * lambda * lambda;
+    }
+    return NAN;
+}
+
+/* ------------------------------------------------------------------------ */
+/* overrelaxation: simulation/monte_carlo/overrelaxation.rs:66-91             */
+/* ------------------------------------------------------------------------ */
+
+/* inverse of a 3x3 complex matrix by cofactors; returns det */
+static or_cplx m3_inverse(const or_mat3* a, or_mat3* inv)
+{
+    or_cplx c[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            c[i][j] = csub(cmul(M(a, i1, j1), M(a, i2, j2)), cmul(M(a, i1, j2), M(a, i2, j1)));
+        }
+    or_cplx det = cadd(cadd(cmul(M(a, 0, 0), c[0][0]), cmul(M(a, 0, 1), c[0][1])), cmul(M(a, 0, 2), c[0][2]));
+    double dd = det.re * det.re + det.im * det.im;
+    or_cplx idet = cx(det.re / dd, -det.im / dd);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) M(inv, i, j) = cmul(c[j][i], idet);
+    return det;
+}
+
+/* Unitary factor W of the polar decomposition A = W P (W = U V^+ of the SVD
+ * A = U S V^+, unique for invertible A), by the scaled Newton iteration
+ * X <- (g X + (g X)^{-+}) / 2, g = |det X|^{-1/3}. */
+void or_polar_unitary(const or_mat3* a, or_mat3* w)
+{
+    or_mat3 x = *a;
+    for (int it = 0; it < 100; ++it) {
+        or_mat3 inv;
+        or_cplx det = m3_inverse(&x, &inv);
+        double g = pow(hypot(det.re, det.im), -1.0 / 3.0);
+        or_mat3 nx;
+        double diff = 0.0, nrm = 0.0;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                /* (inv^+)_ij = conj(inv_ji) */
+                or_cplx t = cconj(M(&inv, j, i));
+                or_cplx v = cx(0.5 * (g * M(&x, i, j).re + t.re / g), 0.5 * (g * M(&x, i, j).im + t.im / g));
+                diff += hypot(v.re - M(&x, i, j).re, v.im - M(&x, i, j).im);
+                nrm += hypot(v.re, v.im);
+                M(&nx, i, j) = v;
+            }
+        x = nx;
+        if (diff <= 1e-15 * nrm) break;
+    }
+    *w = x;
+}
+
+/* OverrelaxationSweepRotation::get_modif (overrelaxation.rs:66-78):
+ * A = staple^+, rot = U V^+ of its SVD, new link = rot U^+ rot */
+static void or_update_link(const or_lattice* l, double* links, size_t site, int dir)
+{
+    const size_t li = site * (size_t)l->dim + (size_t)dir;
+    or_mat3 u, st, a, rot, ud, t, nw;
+    load_mat(links + 18 * li, &u);
+    or_staple(l, links, site, dir, &st);
+    or_adj(&st, &a);
+    or_polar_unitary(&a, &rot);
+    or_adj(&u, &ud);
+    or_matmul(&rot, &ud, &t);
+    or_matmul(&t, &rot, &nw);
+    store_mat(links + 18 * li, &nw);
+}
+
+int or_overrelaxation_sweep(const or_lattice* l, double* links, int order)
+{
+    if (order == OR_ORDER_LEXICOGRAPHIC) {
+        for (size_t s = 0; s < l->nsite; ++s)
+            for (int d = 0; d < l->dim; ++d) or_update_link(l, links, s, d);
+    } else {
+        for (int i = 0; i < l->dim; ++i)
+            if (l->extent[i] % 2) return -2;
+        for (int d = 0; d < l->dim; ++d)
+            for (int par = 0; par < 2; ++par)
+                for (size_t s = 0; s < l->nsite; ++s)
+                    if (or_site_parity(l, s) == par) or_update_link(l, links, s, d);
+    }
+    return 0;
+}
+
 /* LatticeStateDefault::normalize_link_matrices, state.rs:754-756 ->
  * LinkMatrix::normalize, src/field.rs:897-901 */
 void or_normalize_link_matrices(const or_lattice* l, double* links)

@@ -145,6 +145,12 @@ typedef struct {
 int or_metropolis_sweep(const or_lattice* l, double* links, double beta, size_t n_update,
                         const double* proposals, const uint64_t* uniforms, int order,
                         or_mh_stats* st);
+void or_su2_project_unorm(const or_cplx m[4], or_cplx out[4]);
+int or_heat_bath_sweep(const or_lattice* l, double* links, double beta, uint64_t seed,
+                       uint64_t counter, int order, size_t* exhausted);
+double or_heat_bath_x0(double p, uint64_t l, uint64_t n, uint64_t seed);
+void or_polar_unitary(const or_mat3* a, or_mat3* w);
+int or_overrelaxation_sweep(const or_lattice* l, double* links, int order);
 
 void or_normalize_link_matrices(const or_lattice* l, double* links);
 

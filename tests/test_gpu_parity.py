@@ -244,7 +244,7 @@ def test_metropolis_device_stream_rejects_bad_arguments():
     announce("metropolis device stream arguments")
     _, _, st = make_state(4, [4, 4, 4, 4])
     sweep = F.lib().lqcd_metropolis_sweep_random
-    assert sweep(st.handle, 6.0, 8001, 0.1, 1, 0, None) == F.ERR_INVALID_ARGUMENT
+    assert sweep(st.handle, 6.0, 4001, 0.1, 1, 0, None) == F.ERR_INVALID_ARGUMENT
     assert sweep(st.handle, 6.0, 1, 1.5, 1, 0, None) == F.ERR_INVALID_ARGUMENT
     assert sweep(st.handle, 6.0, 0, 0.1, 1, 0, None) == F.ERR_INVALID_ARGUMENT
     assert sweep(st.handle, 6.0, 1, 0.1, 1, 0, None) == F.OK
@@ -408,3 +408,46 @@ def test_gauss_projection_f32_storage_and_limits():
     hmc.next_element(st)
     ef = lqcd.LatticeStateEFSyncDefault.new_random_e_state(st, 11, 0)
     assert ef.gauss_sum_div() <= np.finfo(float).eps * ol.nsite * 320
+
+
+@pytest.mark.parametrize("dim,ext,precision", [(4, [4, 4, 4, 4], lqcd.F64), (3, [4, 6, 4], lqcd.F64),
+                                               (4, [4, 4, 4, 6], lqcd.F32)])
+def test_heat_bath_sweep_device_stream(dim, ext, precision):
+    # HeatBathSweep (heat_bath.rs:73-123), checkerboard order, device stream
+    announce(f"heat bath D={dim} ext={ext} prec={precision}")
+    beta, seed = 5.5, 0xB0A7
+    ol, u, st = make_state(dim, ext, beta, precision=precision)
+    u = st.link_matrix()
+    hb = lqcd.HeatBathSweep(seed)
+    for sweep in range(2):
+        st.monte_carlo_step(hb)
+        u, ex = ol.heat_bath_sweep(u, beta, seed, sweep)
+        assert hb.exhausted_last() == ex == 0
+        got = st.link_matrix()
+        if precision == lqcd.F64:
+            assert np.max(np.abs(got - u)) <= 1e-10
+        else:
+            # f64 arithmetic, f32 storage; the oracle continues from the stored links
+            assert np.max(np.abs(got - u)) <= 1e-4 * max(1.0, np.max(np.abs(u)))
+            u = got
+    assert hb.counter() == 2
+
+
+@pytest.mark.parametrize("dim,ext", [(4, [4, 4, 4, 4]), (3, [6, 4, 4])])
+def test_overrelaxation_sweep(dim, ext):
+    # OverrelaxationSweepRotation (overrelaxation.rs:66-91), checkerboard order
+    announce(f"overrelaxation D={dim} ext={ext}")
+    beta = 1.0
+    ol, u, st = make_state(dim, ext, beta)
+    h = st.hamiltonian_links()
+    st.monte_carlo_step(lqcd.OverrelaxationSweepRotation())
+    ref = ol.overrelaxation_sweep(u, O.ORDER_CHECKERBOARD)
+    assert np.max(np.abs(st.link_matrix() - ref)) <= 1e-11
+    h2 = st.hamiltonian_links()
+    assert abs(h - h2) < np.finfo(float).eps * 100 * ol.nsite * (h + h2) * 0.5
+    # composed as the reference's HybridMethodVec
+    mix = lqcd.HybridMethodVec([lqcd.HeatBathSweep(3), lqcd.OverrelaxationSweepRotation()])
+    st.monte_carlo_step(mix)
+    with pytest.raises(lqcd.LqcdError):
+        _, _, odd = make_state(3, [3, 4, 4])
+        odd.monte_carlo_step(lqcd.OverrelaxationSweepRotation())

@@ -1,0 +1,72 @@
+"""End-to-end GPU parity of the 8^4 FP64 workload against the oracle: the
+whole run (plaquette, HMC trajectories with the Gauss-projected momentum
+refresh of new_random_e_state, Metropolis sweeps with reunitarisation every
+10 sweeps) on identical draws, checked at the end and along the way."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import lqcd
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(msg):
+    print(f"\n[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def test_small_fp64_workload_end_to_end():
+    announce("8^4 FP64 workload vs oracle")
+    L, beta, dt, md, n_traj, n_sweeps, spread = 8, 6.0, 0.01, 20, 10, 100, 0.1
+    seed_mom, seed_mh = 0x1111, 0x2222
+    ol = O.OracleLattice(4, L)
+    u = ol.device_hot_start(O.SEED_RNG)
+    st = lqcd.LatticeStateDefault.new_determinist(1.0, beta, L, O.SEED_RNG, 4, lqcd.F64)
+    assert np.max(np.abs(st.link_matrix() - u)) <= 1e-14
+    p_ref = ol.average_trace_plaquette(u)
+    assert abs(st.average_trace_plaquette() - p_ref) <= 1e-13 * abs(p_ref)
+
+    ef = lqcd.LatticeStateEFSyncDefault(st)
+    hmc = lqcd.HybridMonteCarloDiagnostic(dt, md, lqcd.SymplecticEulerGpu(lqcd.DRIFT_ADDITIVE,
+                                                                        lqcd.SCHEME_REFERENCE), 1)
+    rng = O.OracleRng(77)
+    accepted = 0
+    for n in range(n_traj):
+        # new_random_e_state: N(0, 0.5 / beta), then project_to_gauss
+        ef.reset_e_field(seed_mom, n, 0.5 / beta)
+        it = ef.project_to_gauss(2000)
+        e = ol.device_momenta(seed_mom, n, 0.5 / beta)
+        rc, e, it_ref = ol.project_to_gauss(u, e, 2000)
+        assert rc == 0 and abs(it - it_ref) <= 1
+        assert np.max(np.abs(ef.e_field() - e)) <= 1e-10
+        word = rng.next_u64()
+        got = hmc.trajectory(st, word)
+        u, ref = ol.hmc_trajectory(u, e, beta, dt, md, word, O.DRIFT_ADDITIVE, O.SCHEME_REFERENCE)
+        tol = 1e-9 * abs(ref.h_old) + 1e-10
+        assert abs(got.delta_h - ref.delta_h) <= tol
+        assert got.accepted == ref.accepted
+        accepted += got.accepted
+        # continue both from the device state so rounding does not accumulate
+        err = np.max(np.abs(st.link_matrix() - u))
+        assert err <= 1e-10, (n, err)
+        u = st.link_matrix()
+    assert accepted > 0
+
+    mh = lqcd.MetropolisHastingsSweep(1, spread, seed_mh)
+    for k in range(n_sweeps):
+        props, words = ol.device_mh_inputs(seed_mh, k, spread, 1)
+        mh.next_element(st)
+        u, ref = ol.metropolis_sweep(u, beta, props, words, O.ORDER_CHECKERBOARD)
+        assert mh.number_replace_last() == ref.number_replace_last
+        if (k + 1) % 10 == 0:
+            st.normalize_link_matrices()
+            u = ol.normalize_link_matrices(u)
+            err = np.max(np.abs(st.link_matrix() - u))
+            assert err <= 1e-11, (k, err)
+            u = st.link_matrix()
+    p_end = st.average_trace_plaquette()
+    assert abs(p_end - ol.average_trace_plaquette(u)) <= 1e-13 * abs(p_end)
+    print(f"plaquette {p_ref.real / 3:.6f} -> {p_end.real / 3:.6f}, {accepted}/{n_traj} accepted")
