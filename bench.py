@@ -88,7 +88,7 @@ def run_leg(lqcd, precision, args, steps, warmup, profile):
     L = args.extent
     st = lqcd.LatticeStateDefault.new_cold(1.0, args.beta, L, 4, precision)
     integ = lqcd.SymplecticEulerGpu(lqcd.DRIFT_EXP, lqcd.SCHEME_LEAPFROG)
-    hmc = lqcd.HybridMonteCarloDiagnostic(args.dt, args.md_steps, integ, args.seed)
+    hmc = lqcd.HybridMonteCarloDiagnostic(args.dt, args.md_steps, integ, args.seed, project_gauss=False)
     ctx = st._dev.ctx
 
     def trajectory():
@@ -163,6 +163,29 @@ def cpu_baseline(args):
                    f"additive drift, f64) of the oracle on the {L}^4 lattice, {threads} OpenMP "
                    f"threads, {t:.1f} s"),
     }
+
+
+def gauss_projection_cost(lqcd, args):
+    """One project_to_gauss (field.rs:1265-1294) of fresh momenta on the
+    headline lattice after a trajectory: the refresh step of the reference's
+    HMC, which the timed trajectories leave out."""
+    st = lqcd.LatticeStateDefault.new_cold(1.0, args.beta, args.extent, 4, lqcd.F32)
+    hmc = lqcd.HybridMonteCarloDiagnostic(args.dt, args.md_steps,
+                                          lqcd.SymplecticEulerGpu(lqcd.DRIFT_EXP, lqcd.SCHEME_LEAPFROG),
+                                          args.seed, project_gauss=False)
+    hmc.next_element(st)
+    st.normalize_link_matrices()
+    ef = lqcd.LatticeStateEFSyncDefault(st)
+    ef.reset_e_field(args.seed, 1000, 0.5 / args.beta)
+    st._dev.ctx.synchronize()
+    t0 = time.perf_counter()
+    try:
+        checks = ef.project_to_gauss(2000)
+        out = {"converged": True, "checks": checks}
+    except lqcd.StateInitializationError as exc:
+        out = {"converged": False, "message": str(exc)}
+    out["seconds"] = round(time.perf_counter() - t0, 4)
+    return out
 
 
 def contract_line(args, metric, value, unit, steps, seconds, dtype, config, roofline, cpu, **extra):
@@ -284,7 +307,8 @@ def hmc_leg(lqcd, precision, extents, beta, dt, md_steps, steps, warmup, seed, c
     else:
         st = lqcd.LatticeStateDefault.new_determinist(1.0, beta, extents, seed, d, precision)
     hmc = lqcd.HybridMonteCarloDiagnostic(dt, md_steps, lqcd.SymplecticEulerGpu(lqcd.DRIFT_EXP,
-                                                                               lqcd.SCHEME_LEAPFROG), seed)
+                                                                               lqcd.SCHEME_LEAPFROG), seed,
+                                          project_gauss=False)
     ctx = st._dev.ctx
     for _ in range(warmup):
         hmc.next_element(st)
@@ -383,7 +407,8 @@ def bench_small(args, lqcd):
     st = lqcd.LatticeStateDefault.new_determinist(1.0, beta, L, args.seed, 4, lqcd.F64)
     nlink = st.lattice().number_of_canonical_links_space()
     hmc = lqcd.HybridMonteCarloDiagnostic(dt, md, lqcd.SymplecticEulerGpu(lqcd.DRIFT_ADDITIVE,
-                                                                        lqcd.SCHEME_REFERENCE), args.seed)
+                                                                        lqcd.SCHEME_REFERENCE), args.seed,
+                                          project_gauss=False)
     mh = lqcd.MetropolisHastingsSweep(1, spread, args.seed)
     ctx = st._dev.ctx
     hmc.next_element(st)
@@ -495,6 +520,7 @@ def main():
 
     cpu = cpu_baseline(args) if args.cpu_md_steps > 0 else None
     copy_gbs = stream_copy_gbs(lqcd)
+    gauss = gauss_projection_cost(lqcd, args)
 
     line = {
         "metric": METRIC,
@@ -512,7 +538,8 @@ def main():
         "config": {
             "workload": (f"HMC {args.extent}^4 D=4 beta={args.beta}, cold start, "
                          f"{args.md_steps} leapfrog steps dt={args.dt}, exp link update, "
-                         "FP32 links/E with FP64 dH/plaquette/accept/reunitarisation per trajectory"),
+                         "FP32 links/E with FP64 dH/plaquette/accept/reunitarisation per trajectory, "
+                         "momenta N(0, 0.5/beta) without the Gauss projection (timed separately)"),
             "lattice": [args.extent] * 4,
             "links": nlink,
             "md_steps": args.md_steps,
@@ -528,6 +555,7 @@ def main():
         "dH_f32_minus_f64_first_trajectories": [
             a - b for a, b in zip(mixed["dh"][: len(f64["dh"])], f64["dh"])][:3],
         "roofline": roofline,
+        "gauss_projection": gauss,
         "stream_copy_GB_s": round(copy_gbs, 1),
         "kernels": kernels,
         "cpu_baseline": cpu,

@@ -484,10 +484,11 @@ int lqcd_efield_project_gauss(lqcd_lattice* L, size_t max_iter, size_t* iteratio
     const double threshold = 2.220446049250313e-16 * (double)(L->g.nsite * 4 * 8 * 10);
     size_t it = 0;
     bool converged = false;
+    double div = 0.0;
     for (; it < max_iter; ++it) {
         HIP_TRY(launch_gauss(L, cur, L->result));
         if (int rc = read_results(L, 1)) return rc;
-        const double div = L->host_result[0];
+        div = L->host_result[0];
         if (std::isnan(div)) {
             if (iterations) *iterations = it;
             return fail(LQCD_ERR_GAUSS_PROJECTION, "gauss projection could not finish: deviation is NaN");
@@ -503,9 +504,13 @@ int lqcd_efield_project_gauss(lqcd_lattice* L, size_t max_iter, size_t* iteratio
         }
     }
     if (iterations) *iterations = it;
-    if (!converged)
-        return fail(LQCD_ERR_GAUSS_PROJECTION, "gauss projection could not finish within " +
-                                                   std::to_string(max_iter) + " iterations");
+    if (!converged) {
+        char msg[160];
+        std::snprintf(msg, sizeof(msg),
+                      "gauss projection could not finish within %zu iterations (deviation %.6e, threshold %.6e)",
+                      max_iter, div, threshold);
+        return fail(LQCD_ERR_GAUSS_PROJECTION, msg);
+    }
     HIP_TRY(launch_efield_from_double(L, cur, L->E));
     HIP_TRY(hipStreamSynchronize(L->ctx->stream));
     return LQCD_OK;
