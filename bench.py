@@ -70,6 +70,10 @@ def parse():
                    help="only run the 1 GiB streaming copy (byte-counter calibration)")
     p.add_argument("--workload", default="hmc32",
                    choices=["hmc32", "batched", "large", "mh3d", "small"])
+    p.add_argument("--precision-trajectories", type=int, default=3,
+                   help="trajectories of the FP32-mixed against FP64 error table (0 = skip)")
+    p.add_argument("--gauss-checks", type=int, default=2000,
+                   help="limit of the separately timed Gauss projection (0 = skip it)")
     p.add_argument("--cpu-seconds", type=float, default=10.0,
                    help="target length of the CPU baseline sample of the other workloads")
     return p.parse_args()
@@ -165,6 +169,35 @@ def cpu_baseline(args):
     }
 
 
+def precision_table(lqcd, args, n_traj=3):
+    """FP32-mixed against FP64 on the same trajectories: from the cold start,
+    momenta n of the same device stream, the full MD trajectory of the
+    headline (40 leapfrog steps, exp drift) in both storage precisions; the
+    per-observable differences of the proposals (every trajectory of this
+    workload is rejected, so the kept states would compare trivially)."""
+    rows = {lqcd.F32: [], lqcd.F64: []}
+    for prec in (lqcd.F32, lqcd.F64):
+        for t in range(n_traj):
+            st = lqcd.LatticeStateDefault.new_cold(1.0, args.beta, args.extent, 4, prec)
+            ef = lqcd.LatticeStateEFSyncDefault(st)
+            ef.reset_e_field(args.seed, 10_000 + t, 0.5 / args.beta)
+            h0 = ef.hamiltonian_total()
+            ef.simulate_symplectic_n(lqcd.SymplecticEulerGpu(lqcd.DRIFT_EXP, lqcd.SCHEME_LEAPFROG), args.dt,
+                                     args.md_steps)
+            h1 = ef.hamiltonian_total()
+            rows[prec].append({"h_old": h0, "h_new": h1, "dh": h1 - h0,
+                               "plaquette": st.average_trace_plaquette().real / 3.0,
+                               "e_energy": ef.hamiltonian_efield()})
+    out = {}
+    for key in ("h_old", "h_new", "dh", "plaquette", "e_energy"):
+        diffs = [abs(a[key] - b[key]) for a, b in zip(rows[lqcd.F32], rows[lqcd.F64])]
+        scale = max(abs(b[key]) for b in rows[lqcd.F64])
+        out[key] = {"max_abs_err": max(diffs), "rel_to_f64": max(diffs) / scale if scale else None,
+                    "f64": [b[key] for b in rows[lqcd.F64]]}
+    out["trajectories"] = n_traj
+    return out
+
+
 def gauss_projection_cost(lqcd, args):
     """One project_to_gauss (field.rs:1265-1294) of fresh momenta on the
     headline lattice after a trajectory: the refresh step of the reference's
@@ -180,7 +213,7 @@ def gauss_projection_cost(lqcd, args):
     st._dev.ctx.synchronize()
     t0 = time.perf_counter()
     try:
-        checks = ef.project_to_gauss(2000)
+        checks = ef.project_to_gauss(args.gauss_checks)
         out = {"converged": True, "checks": checks}
     except lqcd.StateInitializationError as exc:
         out = {"converged": False, "message": str(exc)}
@@ -520,7 +553,9 @@ def main():
 
     cpu = cpu_baseline(args) if args.cpu_md_steps > 0 else None
     copy_gbs = stream_copy_gbs(lqcd)
-    gauss = gauss_projection_cost(lqcd, args)
+    gauss = gauss_projection_cost(lqcd, args) if args.gauss_checks > 0 else None
+    precision = (precision_table(lqcd, args, args.precision_trajectories)
+                 if args.precision_trajectories > 0 else None)
 
     line = {
         "metric": METRIC,
@@ -556,6 +591,7 @@ def main():
             a - b for a, b in zip(mixed["dh"][: len(f64["dh"])], f64["dh"])][:3],
         "roofline": roofline,
         "gauss_projection": gauss,
+        "mixed_precision_errors": precision,
         "stream_copy_GB_s": round(copy_gbs, 1),
         "kernels": kernels,
         "cpu_baseline": cpu,

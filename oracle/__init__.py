@@ -125,6 +125,8 @@ def _declare(L):
         "or_heat_bath_x0": (_D, [_D, _U64, _U64, _U64]),
         "or_polar_unitary": (None, [_P, _P]),
         "or_overrelaxation_sweep": (_I, [LP, _P, _I]),
+        "or_svd3": (None, [_P, _P, _P, _P]),
+        "or_overrelaxation_reverse_sweep": (_I, [LP, _P, _I]),
         "or_bernoulli_accept": (_I, [_D, _U64]),
         "or_clamp_probability": (_D, [_D]),
         "or_link_matrix": (None, [LP, _P, _S, _I, _I, _P]),
@@ -376,6 +378,13 @@ class OracleLattice:
             raise ValueError(f"or_overrelaxation_sweep failed: {rc}")
         return u
 
+    def overrelaxation_reverse_sweep(self, u, order=ORDER_CHECKERBOARD):
+        u = np.array(u, copy=True)
+        rc = lib().or_overrelaxation_reverse_sweep(self.ref, _ptr(u), order)
+        if rc != 0:
+            raise ValueError(f"or_overrelaxation_reverse_sweep failed: {rc}")
+        return u
+
     def normalize_link_matrices(self, u):
         u = np.array(u, copy=True)
         lib().or_normalize_link_matrices(self.ref, _ptr(u))
@@ -473,6 +482,16 @@ def polar_unitary(m18) -> np.ndarray:
     out = np.zeros(18)
     lib().or_polar_unitary(_ptr(m), _ptr(out))
     return out
+
+
+def svd3(m18):
+    """(U, singular values, V) of a 3x3 complex matrix (column-major 18)."""
+    m = np.ascontiguousarray(m18, dtype=np.float64)
+    u = np.zeros(18)
+    v = np.zeros(18)
+    sv = np.zeros(3)
+    lib().or_svd3(_ptr(m), _ptr(u), _ptr(sv), _ptr(v))
+    return u, sv, v
 
 
 def philox4x32(ctr, key):

@@ -1477,6 +1477,101 @@ static void or_update_link(const or_lattice* l, double* links, size_t site, int 
     store_mat(links + 18 * li, &nw);
 }
 
+/* SVD A = U S V^+ of a 3x3 complex matrix by one-sided (Hestenes) Jacobi:
+ * rotate column pairs of A (and of V = I) until they are orthogonal; then
+ * S_k = |a_k| and U_k = a_k / S_k.  Columns with S_k = 0 are left as they
+ * are (the overrelaxation input is nonsingular). */
+void or_svd3(const or_mat3* a, or_mat3* u, double sv[3], or_mat3* v)
+{
+    or_mat3 w = *a;
+    mident(v);
+    for (int sweep = 0; sweep < 30; ++sweep) {
+        double off = 0.0, diag = 0.0;
+        for (int p = 0; p < 2; ++p)
+            for (int q = p + 1; q < 3; ++q) {
+                double al = 0.0, be = 0.0;
+                or_cplx ga = C_ZERO; /* a_p^+ a_q */
+                for (int i = 0; i < 3; ++i) {
+                    or_cplx x = M(&w, i, p), y = M(&w, i, q);
+                    al += x.re * x.re + x.im * x.im;
+                    be += y.re * y.re + y.im * y.im;
+                    ga = cadd(ga, cmul(cconj(x), y));
+                }
+                double g = hypot(ga.re, ga.im);
+                off += g;
+                diag += sqrt(al * be);
+                if (g <= 1e-300) continue;
+                /* phase e^{i phi} = ga / g; real rotation of [[al, g], [g, be]] */
+                or_cplx ph = cx(ga.re / g, ga.im / g);
+                double zeta = (be - al) / (2.0 * g);
+                double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
+                /* a_p' = c a_p - s conj(ph) a_q ... in the form that zeroes a_p'^+ a_q' */
+                for (int i = 0; i < 3; ++i) {
+                    or_cplx x = M(&w, i, p), y = M(&w, i, q);
+                    or_cplx yp = cmul(y, cconj(ph));
+                    M(&w, i, p) = cx(c * x.re - sn * yp.re, c * x.im - sn * yp.im);
+                    or_cplx xq = cmul(x, ph);
+                    M(&w, i, q) = cx(sn * xq.re + c * y.re, sn * xq.im + c * y.im);
+                    or_cplx vx = M(v, i, p), vy = M(v, i, q);
+                    or_cplx vyp = cmul(vy, cconj(ph));
+                    M(v, i, p) = cx(c * vx.re - sn * vyp.re, c * vx.im - sn * vyp.im);
+                    or_cplx vxq = cmul(vx, ph);
+                    M(v, i, q) = cx(sn * vxq.re + c * vy.re, sn * vxq.im + c * vy.im);
+                }
+            }
+        if (off <= 1e-16 * diag) break;
+    }
+    for (int k = 0; k < 3; ++k) {
+        double n = 0.0;
+        for (int i = 0; i < 3; ++i) n += M(&w, i, k).re * M(&w, i, k).re + M(&w, i, k).im * M(&w, i, k).im;
+        n = sqrt(n);
+        sv[k] = n;
+        for (int i = 0; i < 3; ++i) M(u, i, k) = n > 0 ? cx(M(&w, i, k).re / n, M(&w, i, k).im / n) : M(&w, i, k);
+    }
+}
+
+/* OverrelaxationSweepReverse::get_modif (overrelaxation.rs:136-150):
+ * A = staple^+ = U S V^+, new link = U reverse(U^+ L V) V^+ with
+ * su3::reverse (su3.rs:705-714) negating the off-diagonal entries.  The
+ * result does not depend on the SVD's column phases or order. */
+static void or_reverse_update_link(const or_lattice* l, double* links, size_t site, int dir)
+{
+    const size_t li = site * (size_t)l->dim + (size_t)dir;
+    or_mat3 lk, st, a, u, v, ud, t1, m, t2, vd, nw;
+    double sv[3];
+    load_mat(links + 18 * li, &lk);
+    or_staple(l, links, site, dir, &st);
+    or_adj(&st, &a);
+    or_svd3(&a, &u, sv, &v);
+    or_adj(&u, &ud);
+    or_matmul(&ud, &lk, &t1);
+    or_matmul(&t1, &v, &m);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            if (i != j) M(&m, i, j) = cx(-M(&m, i, j).re, -M(&m, i, j).im);
+    or_matmul(&u, &m, &t2);
+    or_adj(&v, &vd);
+    or_matmul(&t2, &vd, &nw);
+    store_mat(links + 18 * li, &nw);
+}
+
+int or_overrelaxation_reverse_sweep(const or_lattice* l, double* links, int order)
+{
+    if (order == OR_ORDER_LEXICOGRAPHIC) {
+        for (size_t s = 0; s < l->nsite; ++s)
+            for (int d = 0; d < l->dim; ++d) or_reverse_update_link(l, links, s, d);
+    } else {
+        for (int i = 0; i < l->dim; ++i)
+            if (l->extent[i] % 2) return -2;
+        for (int d = 0; d < l->dim; ++d)
+            for (int par = 0; par < 2; ++par)
+                for (size_t s = 0; s < l->nsite; ++s)
+                    if (or_site_parity(l, s) == par) or_reverse_update_link(l, links, s, d);
+    }
+    return 0;
+}
+
 int or_overrelaxation_sweep(const or_lattice* l, double* links, int order)
 {
     if (order == OR_ORDER_LEXICOGRAPHIC) {

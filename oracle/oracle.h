@@ -151,6 +151,8 @@ int or_heat_bath_sweep(const or_lattice* l, double* links, double beta, uint64_t
 double or_heat_bath_x0(double p, uint64_t l, uint64_t n, uint64_t seed);
 void or_polar_unitary(const or_mat3* a, or_mat3* w);
 int or_overrelaxation_sweep(const or_lattice* l, double* links, int order);
+void or_svd3(const or_mat3* a, or_mat3* u, double sv[3], or_mat3* v);
+int or_overrelaxation_reverse_sweep(const or_lattice* l, double* links, int order);
 
 void or_normalize_link_matrices(const or_lattice* l, double* links);
 

@@ -89,3 +89,43 @@ def test_overrelaxation_keeps_the_action(order):
     m = O.to_np(u2)
     assert np.max(np.abs(m @ np.conj(np.swapaxes(m, -1, -2)) - np.eye(3))) < 1e-12
     assert not np.allclose(u2, u)
+
+
+def test_svd3_reconstructs():
+    rng = np.random.default_rng(3)
+    for _ in range(100):
+        a = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
+        u, sv, v = O.svd3(O.from_np(a))
+        uu, vv = O.to_np(u), O.to_np(v)
+        assert np.max(np.abs(uu @ np.diag(sv) @ vv.conj().T - a)) < 1e-13
+        assert np.allclose(np.sort(sv), np.sort(np.linalg.svd(a, compute_uv=False)), atol=1e-12)
+
+
+def test_overrelaxation_reverse_matches_lapack_svd_sweep():
+    # OverrelaxationSweepReverse::get_modif (overrelaxation.rs:136-150):
+    # U reverse(U^+ L V) V^+ for staple^+ = U S V^+; a checkerboard sweep rebuilt
+    # with numpy's LAPACK SVD (other phase / order conventions) must agree.  (In
+    # lexicographic order every update depends on the previous one and rounding
+    # differences grow along the sweep; the checkerboard has 6 dependent stages.)
+    l = O.OracleLattice(3, 4)
+    u0 = l.hot_start(O.SEED_RNG)
+    ref = u0.copy()
+    for d in range(3):
+        for par in range(2):
+            new = {}
+            for s in range(l.nsite):
+                if l.parity(s) != par:
+                    continue
+                li = s * 3 + d
+                a = O.to_np(l.staple(ref, s, d)).conj().T
+                uu, _, vh = np.linalg.svd(a)
+                m = uu.conj().T @ O.to_np(ref[li]) @ vh.conj().T
+                m = np.where(np.eye(3, dtype=bool), m, -m)
+                new[li] = O.from_np(uu @ m @ vh)
+            for li, val in new.items():
+                ref[li] = val
+    got = l.overrelaxation_reverse_sweep(u0, O.ORDER_CHECKERBOARD)
+    assert np.max(np.abs(got - ref)) < 1e-11
+    # overrelaxation.rs test_same_energy criterion
+    h, h2 = l.hamiltonian_links(u0, 1.0), l.hamiltonian_links(got, 1.0)
+    assert abs(h - h2) < np.finfo(float).eps * 100 * 4 ** 3 * (h + h2) * 0.5
