@@ -222,6 +222,13 @@ int lqcd_heat_bath_sweep(lqcd_lattice* lat, double beta, uint64_t seed, uint64_t
  * as the unitary polar factor (scaled Newton iteration, double arithmetic).
  * Leaves the Wilson action unchanged.  Even extents. */
 int lqcd_overrelaxation_sweep(lqcd_lattice* lat);
+/* OverrelaxationSweepReverse::next_element
+ * (src/simulation/monte_carlo/overrelaxation.rs:136-183) in checkerboard
+ * order: with staple^+ = W S V^+ (one-sided Jacobi SVD, double arithmetic),
+ * new link = W reverse(W^+ U V) V^+, su3::reverse (src/su3.rs:705-714)
+ * negating the off-diagonal entries.  Leaves the Wilson action unchanged.
+ * Even extents. */
+int lqcd_overrelaxation_reverse_sweep(lqcd_lattice* lat);
 
 /* LatticeStateDefault::normalize_link_matrices (src/simulation/state.rs:754-756) */
 int lqcd_normalize_link_matrices(lqcd_lattice* lat);

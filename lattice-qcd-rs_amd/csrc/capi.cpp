@@ -729,17 +729,21 @@ int lqcd_heat_bath_sweep(lqcd_lattice* L, double beta, uint64_t seed, uint64_t c
     return LQCD_OK;
 }
 
-int lqcd_overrelaxation_sweep(lqcd_lattice* L)
+static int overrelaxation(lqcd_lattice* L, bool reverse)
 {
     CHECK_ARG(L, "lattice is null");
     for (int i = 0; i < L->g.dim; ++i)
         if (L->g.extent[i] % 2) return fail(LQCD_ERR_UNSUPPORTED, "checkerboard sweep needs even extents");
     DeviceGuard dg(L->ctx->device);
     for (int mu = 0; mu < L->g.dim; ++mu)
-        for (int par = 0; par < 2; ++par) HIP_TRY(launch_overrelax_class(L, L->U[L->cur], mu, par));
+        for (int par = 0; par < 2; ++par) HIP_TRY(launch_overrelax_class(L, L->U[L->cur], mu, par, reverse));
     HIP_TRY(hipStreamSynchronize(L->ctx->stream));
     return LQCD_OK;
 }
+
+int lqcd_overrelaxation_sweep(lqcd_lattice* L) { return overrelaxation(L, false); }
+
+int lqcd_overrelaxation_reverse_sweep(lqcd_lattice* L) { return overrelaxation(L, true); }
 
 int lqcd_normalize_link_matrices(lqcd_lattice* L)
 {

@@ -21,6 +21,13 @@ namespace lqcd {
 // descriptor word 3 of a raw buffer resource on the gfx9 family
 constexpr int kBufferFlags = 0x00020000;
 
+// cache-policy operand of a buffer access on gfx950: 0 is the default; bit 1
+// (nt) marks the line non-temporal, so the L2 evicts it first.  Fields that a
+// kernel reads or writes exactly once (E, the new links) use kCacheStream and
+// leave the L2 to the neighbour links that other sites read again.
+constexpr int kCacheDefault = 0;
+constexpr int kCacheStream = 2;
+
 template <typename S>
 struct vec2;
 template <>
@@ -42,31 +49,31 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t planes_rsrc(const E* p, int64_
                                              kBufferFlags);
 }
 
-template <typename E>
+template <typename E, int CP = kCacheDefault>
 __device__ __forceinline__ E buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff)
 {
     if constexpr (sizeof(E) == 4) {
-        return __builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, (int)soff, 0));
+        return __builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, (int)soff, CP));
     } else if constexpr (sizeof(E) == 8) {
-        return __builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, (int)soff, 0));
+        return __builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, (int)soff, CP));
     } else {
         static_assert(sizeof(E) == 16, "element of 4, 8 or 16 bytes");
-        return __builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, (int)soff, 0));
+        return __builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, (int)soff, CP));
     }
 }
 
-template <typename E>
+template <typename E, int CP = kCacheDefault>
 __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff, E v)
 {
     if constexpr (sizeof(E) == 4) {
-        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)off, (int)soff, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)off, (int)soff, CP);
     } else if constexpr (sizeof(E) == 8) {
         using V = __attribute__((ext_vector_type(2))) uint32_t;
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, v), r, (int)off, (int)soff, 0);
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, v), r, (int)off, (int)soff, CP);
     } else {
         static_assert(sizeof(E) == 16, "element of 4, 8 or 16 bytes");
         using V = __attribute__((ext_vector_type(4))) uint32_t;
-        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V, v), r, (int)off, (int)soff, 0);
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V, v), r, (int)off, (int)soff, CP);
     }
 }
 
@@ -87,7 +94,7 @@ __device__ __forceinline__ mat3<T> load_link(const vec2_t<S>* __restrict__ U, in
     return m;
 }
 
-template <typename T, typename S>
+template <typename T, typename S, int CP = kCacheDefault>
 __device__ __forceinline__ void store_link(vec2_t<S>* __restrict__ U, int64_t npad, int mu, int64_t s,
                                            const mat3<T>& m)
 {
@@ -100,28 +107,28 @@ __device__ __forceinline__ void store_link(vec2_t<S>* __restrict__ U, int64_t np
         E v;
         v.x = S(m.e[k].re);
         v.y = S(m.e[k].im);
-        buf_store<E>(r, off, (uint32_t)k * plane, v);
+        buf_store<E, CP>(r, off, (uint32_t)k * plane, v);
     }
 }
 
-template <typename T, typename S>
+template <typename T, typename S, int CP = kCacheDefault>
 __device__ __forceinline__ void load_e(const S* __restrict__ E, int64_t npad, int mu, int64_t s, T* e)
 {
     const __amdgpu_buffer_rsrc_t r = planes_rsrc(E + (int64_t)mu * 8 * npad, npad, 8);
     const uint32_t off = (uint32_t)s * (uint32_t)sizeof(S);
     const uint32_t plane = (uint32_t)npad * (uint32_t)sizeof(S);
 #pragma unroll
-    for (int a = 0; a < 8; ++a) e[a] = T(buf_load<S>(r, off, (uint32_t)a * plane));
+    for (int a = 0; a < 8; ++a) e[a] = T(buf_load<S, CP>(r, off, (uint32_t)a * plane));
 }
 
-template <typename T, typename S>
+template <typename T, typename S, int CP = kCacheDefault>
 __device__ __forceinline__ void store_e(S* __restrict__ E, int64_t npad, int mu, int64_t s, const T* e)
 {
     const __amdgpu_buffer_rsrc_t r = planes_rsrc(E + (int64_t)mu * 8 * npad, npad, 8);
     const uint32_t off = (uint32_t)s * (uint32_t)sizeof(S);
     const uint32_t plane = (uint32_t)npad * (uint32_t)sizeof(S);
 #pragma unroll
-    for (int a = 0; a < 8; ++a) buf_store<S>(r, off, (uint32_t)a * plane, S(e[a]));
+    for (int a = 0; a < 8; ++a) buf_store<S, CP>(r, off, (uint32_t)a * plane, S(e[a]));
 }
 
 } // namespace lqcd

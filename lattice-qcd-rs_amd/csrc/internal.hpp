@@ -118,7 +118,8 @@ hipError_t launch_metropolis_class_rng(const lqcd_lattice* L, void* U, int mu, i
 // bath writes the per-link count of exhausted samplers to mh_acc (mh_p = 0)
 hipError_t launch_heat_bath_class(const lqcd_lattice* L, void* U, int mu, int parity, double beta, uint64_t seed,
                                   uint64_t counter);
-hipError_t launch_overrelax_class(const lqcd_lattice* L, void* U, int mu, int parity);
+// (reverse: OverrelaxationSweepReverse, else OverrelaxationSweepRotation)
+hipError_t launch_overrelax_class(const lqcd_lattice* L, void* U, int mu, int parity, bool reverse);
 
 // ---- Gauss law (k_gauss.hip); links are U[cur], G goes to gauss_g ----
 // G(x) of the double E copy Ed, and div = sum_x |Tr(sum_a T^a G(x))| into d_div

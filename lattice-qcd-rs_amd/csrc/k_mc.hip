@@ -385,24 +385,104 @@ __device__ __forceinline__ void polar_unitary(const mat3<double>& a, mat3<double
     }
 }
 
+// Singular vectors of A = U S V^+ by one-sided Jacobi: plane rotations of
+// column pairs (p, q) applied to W = A V until every pair is orthogonal; then
+// the columns of W are U S.  Each rotation removes the Hermitian inner product
+// w_p^+ w_q of the pair: with w_p^+ w_q = g e^{i phi}, the 2x2 Gram matrix
+// [[|w_p|^2, g], [g, |w_q|^2]] is diagonalised by the real Jacobi angle t and
+// the phase e^{i phi} moved onto column q.
+__device__ __forceinline__ void svd3_jacobi(const mat3<double>& a, mat3<double>& u, mat3<double>& v)
+{
+    mat3<double> w = a;
+    set_identity(v);
+    for (int sweep = 0; sweep < 30; ++sweep) {
+        double off = 0.0, diag = 0.0;
+#pragma unroll
+        for (int pq = 0; pq < 3; ++pq) {
+            const int p = pq == 2 ? 1 : 0, q = pq == 0 ? 1 : 2;
+            double al = 0.0, be = 0.0;
+            cplx<double> ga = {0.0, 0.0};
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const cplx<double> x = w.e[i * 3 + p], y = w.e[i * 3 + q];
+                al += x.re * x.re + x.im * x.im;
+                be += y.re * y.re + y.im * y.im;
+                ga.re += x.re * y.re + x.im * y.im;
+                ga.im += x.re * y.im - x.im * y.re;
+            }
+            const double gn = hypot(ga.re, ga.im);
+            off += gn;
+            diag += sqrt(al * be);
+            if (gn <= 1e-300) continue;
+            const cplx<double> ph = {ga.re / gn, ga.im / gn};
+            const double zeta = (be - al) / (2.0 * gn);
+            const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+            const double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const cplx<double> x = w.e[i * 3 + p], y = w.e[i * 3 + q];
+                const cplx<double> yp = {y.re * ph.re + y.im * ph.im, y.im * ph.re - y.re * ph.im};
+                const cplx<double> xq = {x.re * ph.re - x.im * ph.im, x.re * ph.im + x.im * ph.re};
+                w.e[i * 3 + p] = {c * x.re - sn * yp.re, c * x.im - sn * yp.im};
+                w.e[i * 3 + q] = {sn * xq.re + c * y.re, sn * xq.im + c * y.im};
+                const cplx<double> vx = v.e[i * 3 + p], vy = v.e[i * 3 + q];
+                const cplx<double> vyp = {vy.re * ph.re + vy.im * ph.im, vy.im * ph.re - vy.re * ph.im};
+                const cplx<double> vxq = {vx.re * ph.re - vx.im * ph.im, vx.re * ph.im + vx.im * ph.re};
+                v.e[i * 3 + p] = {c * vx.re - sn * vyp.re, c * vx.im - sn * vyp.im};
+                v.e[i * 3 + q] = {sn * vxq.re + c * vy.re, sn * vxq.im + c * vy.im};
+            }
+        }
+        if (off <= 1e-16 * diag) break;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double n = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) n += w.e[i * 3 + k].re * w.e[i * 3 + k].re + w.e[i * 3 + k].im * w.e[i * 3 + k].im;
+        n = sqrt(n);
+        const double r = n > 0.0 ? 1.0 / n : 1.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) u.e[i * 3 + k] = {w.e[i * 3 + k].re * r, w.e[i * 3 + k].im * r};
+    }
+}
+
 // OverrelaxationSweepRotation::get_modif (overrelaxation.rs:66-78):
-// rot = polar factor of staple^+, new link = rot U^+ rot
-template <int D, typename S>
+//   rot = polar factor of staple^+, new link = rot U^+ rot;
+// OverrelaxationSweepReverse::get_modif (overrelaxation.rs:136-150):
+//   staple^+ = W S V^+, new link = W reverse(W^+ U V) V^+, where su3::reverse
+//   (su3.rs:705-714) negates the off-diagonal entries.  The result does not
+//   depend on the phases or the order of the singular vectors.
+template <int D, typename S, bool REVERSE>
 __global__ __launch_bounds__(kBlock) void k_overrelax(vec2_t<S>* __restrict__ U, Geom g, int mu, int parity)
 {
     const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (k >= g.nsite / 2) return;
     const int64_t s = class_site<D>(g, k, parity);
-    mat3<double> a, ad, rot, t, nw;
+    mat3<double> a, ad, t, nw;
     mc_staple<D, S>(U, g, mu, s, a);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) ad.e[i * 3 + j] = {a.e[j * 3 + i].re, -a.e[j * 3 + i].im};
-    polar_unitary(ad, rot);
     const mat3<double> u = load_link<double, S>(U, g.npad, mu, s);
-    mul<false, true>(rot, u, t);
-    mul<false, false>(t, rot, nw);
+    if (REVERSE) {
+        mat3<double> w, v, m;
+        svd3_jacobi(ad, w, v);
+        mul<true, false>(w, u, t);
+        mul<false, false>(t, v, m);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                if (i != j) m.e[i * 3 + j] = {-m.e[i * 3 + j].re, -m.e[i * 3 + j].im};
+        mul<false, false>(w, m, t);
+        mul<false, true>(t, v, nw);
+    } else {
+        mat3<double> rot;
+        polar_unitary(ad, rot);
+        mul<false, true>(rot, u, t);
+        mul<false, false>(t, rot, nw);
+    }
     store_link<double, S>(U, g.npad, mu, s, nw);
 }
 
@@ -418,10 +498,15 @@ hipError_t launch_heat_bath_class(const lqcd_lattice* L, void* U, int mu, int pa
     return hipGetLastError();
 }
 
-hipError_t launch_overrelax_class(const lqcd_lattice* L, void* U, int mu, int parity)
+hipError_t launch_overrelax_class(const lqcd_lattice* L, void* U, int mu, int parity, bool reverse)
 {
     const unsigned grid = grid_for(L->g.nsite / 2);
-    LQCD_DISPATCH_DS(L, k_overrelax<D, S><<<grid, kBlock, 0, L->ctx->stream>>>((vec2_t<S>*)U, L->g, mu, parity));
+    if (reverse)
+        LQCD_DISPATCH_DS(L, k_overrelax<D, S, true><<<grid, kBlock, 0, L->ctx->stream>>>((vec2_t<S>*)U, L->g,
+                                                                                         mu, parity));
+    else
+        LQCD_DISPATCH_DS(L, k_overrelax<D, S, false><<<grid, kBlock, 0, L->ctx->stream>>>((vec2_t<S>*)U, L->g,
+                                                                                          mu, parity));
     return hipGetLastError();
 }
 

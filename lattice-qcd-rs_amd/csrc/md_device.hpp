@@ -144,14 +144,14 @@ __device__ __forceinline__ void finish_link(const mat3<T>& u, const mat3<T>& sig
     mul<false, false>(u, sigma, m);
     T f[8], e[8];
     im_trace_generators(m, f);
-    load_e<T, S>(E, g.npad, mu, s, e);
+    load_e<T, S, kCacheStream>(E, g.npad, mu, s, e);
 #pragma unroll
     for (int a = 0; a < 8; ++a) e[a] = fma(kick, fscale * f[a], e[a]);
-    store_e<T, S>(E, g.npad, mu, s, e);
+    store_e<T, S, kCacheStream>(E, g.npad, mu, s, e);
     if (DR >= 0) {
         mat3<T> un;
         drift_link<T, DR>(u, e, dfac, un);
-        store_link<T, S>(U_out, g.npad, mu, s, un);
+        store_link<T, S, kCacheStream>(U_out, g.npad, mu, s, un);
     }
 }
 
@@ -192,10 +192,10 @@ __device__ __forceinline__ void md_site(const vec2_t<S>* __restrict__ U_in, vec2
     T e[D][8];
 #pragma unroll
     for (int mu = 0; mu < D; ++mu) {
-        load_e<T, S>(E, g.npad, mu, s, e[mu]);
+        load_e<T, S, kCacheStream>(E, g.npad, mu, s, e[mu]);
 #pragma unroll
         for (int a = 0; a < 8; ++a) e[mu][a] = fma(kick, fscale * f[mu][a], e[mu][a]);
-        store_e<T, S>(E, g.npad, mu, s, e[mu]);
+        store_e<T, S, kCacheStream>(E, g.npad, mu, s, e[mu]);
     }
     // phase 3: drifts (U_mu(x) is read again, from cache)
     if (DR >= 0) {
@@ -204,7 +204,7 @@ __device__ __forceinline__ void md_site(const vec2_t<S>* __restrict__ U_in, vec2
             const mat3<T> u = load_link<T, S>(U_in, g.npad, mu, s);
             mat3<T> un;
             drift_link<T, DR>(u, e[mu], dfac, un);
-            store_link<T, S>(U_out, g.npad, mu, s, un);
+            store_link<T, S, kCacheStream>(U_out, g.npad, mu, s, un);
         }
     }
 }

@@ -15,6 +15,7 @@ reference (lattice-qcd-rs)              here
 ``MetropolisHastingsSweep``             ``MetropolisHastingsSweep`` (checkerboard)
 ``HeatBathSweep``                       ``HeatBathSweep`` (checkerboard, device stream)
 ``OverrelaxationSweepRotation``         ``OverrelaxationSweepRotation`` (checkerboard)
+``OverrelaxationSweepReverse``          ``OverrelaxationSweepReverse`` (checkerboard)
 ``HybridMethodVec``                     ``HybridMethodVec``
 independent chains, one after another   ``BatchedChains`` (one launch per MD step)
 =====================================  ==========================================
@@ -37,7 +38,8 @@ from ._ffi import (DRIFT_ADDITIVE, DRIFT_EXP, F32, F64, SCHEME_LEAPFROG, SCHEME_
 __all__ = [
     "Context", "LatticeCyclic", "LatticeStateDefault", "LatticeStateEFSyncDefault",
     "SymplecticEulerGpu", "HybridMonteCarloDiagnostic", "MetropolisHastingsSweep",
-    "BatchedChains", "ChainSpec", "HeatBathSweep", "OverrelaxationSweepRotation", "HybridMethodVec",
+    "BatchedChains", "ChainSpec", "HeatBathSweep", "OverrelaxationSweepRotation",
+    "OverrelaxationSweepReverse", "HybridMethodVec",
     "LatticeInitializationError", "StateInitializationError", "MultiIntegrationError",
     "LqcdError", "F32", "F64", "DRIFT_ADDITIVE", "DRIFT_EXP", "SCHEME_REFERENCE",
     "SCHEME_LEAPFROG",
@@ -725,6 +727,15 @@ class OverrelaxationSweepRotation:
 
     def next_element(self, state: LatticeStateDefault) -> LatticeStateDefault:
         _raise(F.lib().lqcd_overrelaxation_sweep(state.handle))
+        return state
+
+
+class OverrelaxationSweepReverse:
+    """``OverrelaxationSweepReverse`` (src/simulation/monte_carlo/overrelaxation.rs:108-183)
+    on the device, checkerboard order; the Wilson action is unchanged."""
+
+    def next_element(self, state: LatticeStateDefault) -> LatticeStateDefault:
+        _raise(F.lib().lqcd_overrelaxation_reverse_sweep(state.handle))
         return state
 
 

@@ -126,6 +126,7 @@ SIGNATURES = {
     "lqcd_metropolis_sweep_random": (_I, [_P, _D, _S, _D, _U64, _U64, C.POINTER(MhStats)]),
     "lqcd_heat_bath_sweep": (_I, [_P, _D, _U64, _U64, C.POINTER(C.c_size_t)]),
     "lqcd_overrelaxation_sweep": (_I, [_P]),
+    "lqcd_overrelaxation_reverse_sweep": (_I, [_P]),
     "lqcd_normalize_link_matrices": (_I, [_P]),
     "lqcd_stream_copy": (_I, [_P, _S, _I, C.POINTER(_D)]),
     "lqcd_md_autotune": (_I, [_P, _I, C.POINTER(C.c_int)]),

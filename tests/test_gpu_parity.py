@@ -451,3 +451,26 @@ def test_overrelaxation_sweep(dim, ext):
     with pytest.raises(lqcd.LqcdError):
         _, _, odd = make_state(3, [3, 4, 4])
         odd.monte_carlo_step(lqcd.OverrelaxationSweepRotation())
+
+
+@pytest.mark.parametrize("dim,ext", [(4, [4, 4, 4, 4]), (3, [6, 4, 4])])
+def test_overrelaxation_reverse_sweep(dim, ext):
+    # OverrelaxationSweepReverse (overrelaxation.rs:136-183), checkerboard order,
+    # device Jacobi SVD against the oracle's
+    announce(f"overrelaxation reverse D={dim} ext={ext}")
+    beta = 1.0
+    ol, u, st = make_state(dim, ext, beta)
+    h = st.hamiltonian_links()
+    for _ in range(2):
+        st.monte_carlo_step(lqcd.OverrelaxationSweepReverse())
+        u = ol.overrelaxation_reverse_sweep(u, O.ORDER_CHECKERBOARD)
+        assert np.max(np.abs(st.link_matrix() - u)) <= 1e-11
+        u = st.link_matrix()
+    h2 = st.hamiltonian_links()
+    assert abs(h - h2) < np.finfo(float).eps * 100 * ol.nsite * (h + h2) * 0.5
+    mix = lqcd.HybridMethodVec([lqcd.HeatBathSweep(5), lqcd.OverrelaxationSweepReverse()])
+    st.monte_carlo_step(mix)
+    assert np.all(np.isfinite(st.link_matrix()))
+    with pytest.raises(lqcd.LqcdError):
+        _, _, odd = make_state(3, [3, 4, 4])
+        odd.monte_carlo_step(lqcd.OverrelaxationSweepReverse())
