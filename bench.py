@@ -72,7 +72,7 @@ def parse():
                    choices=["hmc32", "batched", "large", "mh3d", "small"])
     p.add_argument("--precision-trajectories", type=int, default=3,
                    help="trajectories of the FP32-mixed against FP64 error table (0 = skip)")
-    p.add_argument("--gauss-checks", type=int, default=2000,
+    p.add_argument("--gauss-checks", type=int, default=4000,
                    help="limit of the separately timed Gauss projection (0 = skip it)")
     p.add_argument("--cpu-seconds", type=float, default=10.0,
                    help="target length of the CPU baseline sample of the other workloads")
