@@ -69,7 +69,7 @@ def parse():
     p.add_argument("--copy-only", action="store_true",
                    help="only run the 1 GiB streaming copy (byte-counter calibration)")
     p.add_argument("--workload", default="hmc32",
-                   choices=["hmc32", "batched", "large", "mh3d", "small"])
+                   choices=["hmc32", "batched", "large", "mh3d", "small", "fields"])
     p.add_argument("--precision-trajectories", type=int, default=3,
                    help="trajectories of the FP32-mixed against FP64 error table (0 = skip)")
     p.add_argument("--gauss-checks", type=int, default=4000,
@@ -433,6 +433,55 @@ def bench_mh3d(args, lqcd):
         number_replace_last=mh.number_replace_last())
 
 
+def bench_fields(args, lqcd):
+    """F_mu_nu (LinkMatrix::f_mu_nu, field.rs:822-834) of every site of the 32^4
+    f32 lattice into a device buffer: the clover stencil of the observables."""
+    F = lqcd._ffi
+    L = args.extent
+    st = lqcd.LatticeStateDefault.new_determinist(1.0, args.beta, L, args.seed, 4, lqcd.F32)
+    nsite = st.lattice().number_of_points()
+    n = nsite * 16 * 18
+    buf = F.DeviceBuffer(8 * n)
+    ctx = st._dev.ctx
+
+    def step():
+        lqcd._raise(F.lib().lqcd_field_strength_device(st.handle, F.FIELD_F_MU_NU, buf.ptr, n))
+
+    for _ in range(args.warmup):
+        step()
+    ctx.profile_enable(1 << F.KERNEL_FIELDS)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.synchronize()
+    seconds = time.perf_counter() - t0
+    # compulsory: the 4 links of a site (72 B each in f32) read, 16 matrices of 144 B written
+    per_launch = nsite * (4 * 72 + 16 * 144)
+    rate = kernel_rate(ctx, F.KERNEL_FIELDS, per_launch)
+    ctx.profile_enable(0)
+    cfg = {"workload": ("F_mu_nu at every site of a 32^4 D=4 lattice (f32 links, double arithmetic, "
+                        "16 matrices per site in the reference's host layout, into device memory)"),
+           "lattice": [L] * 4, "sites": nsite}
+    cpu = None
+    if args.cpu_md_steps > 0:
+        import oracle as O
+        ol = O.OracleLattice(4, 16)
+        u = ol.hot_start(args.seed)
+        O.set_threads(16)
+        t0 = time.perf_counter()
+        k = 0
+        while time.perf_counter() - t0 < min(args.cpu_seconds, 10.0):
+            ol.f_mu_nu_field(u)
+            k += 1
+        t = time.perf_counter() - t0
+        cpu = {"value": ol.nsite * k / t, "unit": "sites/s", "cores": 16, "kind": "port",
+               "sample": f"{k} F_mu_nu fields of the oracle on 16^4, 16 OpenMP threads, {t:.1f} s"}
+    return contract_line(
+        args, "F_mu_nu sites/s, 32^4", nsite * args.steps / seconds, "sites/s", args.steps, seconds, "f64",
+        cfg, roofline_from(rate, "k_clover_fields<4,float,1> (clover, F_mu_nu)"), cpu)
+
+
 def bench_small(args, lqcd):
     F = lqcd._ffi
     L, beta, spread = 8, 6.0, 0.1
@@ -506,7 +555,7 @@ def main():
         return
     if args.workload != "hmc32":
         run = {"batched": bench_batched, "large": bench_large, "mh3d": bench_mh3d,
-               "small": bench_small}[args.workload]
+               "small": bench_small, "fields": bench_fields}[args.workload]
         print(json.dumps(run(args, lqcd)))
         return
 

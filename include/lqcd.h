@@ -230,6 +230,25 @@ int lqcd_overrelaxation_sweep(lqcd_lattice* lat);
  * Even extents. */
 int lqcd_overrelaxation_reverse_sweep(lqcd_lattice* lat);
 
+/* ------------------------------------------------------- field strength */
+/* LinkMatrix::clover (src/field.rs:806-820) at every site for every ordered
+ * pair (i, j) of positive directions, i == j included: out[(site*D + i)*D + j]
+ * is clover(x, i, j), 18 doubles (column-major 3x3 complex, the layout of one
+ * link).  n = nsite * D * D * 18.  Double arithmetic on either storage type. */
+int lqcd_clover_field(lqcd_lattice* lat, double* out, size_t n);
+/* LinkMatrix::f_mu_nu (src/field.rs:822-834), (clover(i,j) - clover(j,i)) /
+ * (8 a^2), same layout and size as lqcd_clover_field. */
+int lqcd_f_mu_nu_field(lqcd_lattice* lat, double* out, size_t n);
+/* LinkMatrix::magnetic_field_vec (src/field.rs:836-868) at every site:
+ * out[site*D + dir] = sum_{i,j} levi_civita(dir, i, j) F_ij / (2i), 18 doubles
+ * each, n = nsite * D * 18 (zero for D = 2). */
+int lqcd_magnetic_field_vec(lqcd_lattice* lat, double* out, size_t n);
+/* The same three fields into a caller-owned device buffer (d_out: device
+ * memory of the lattice's GPU, n doubles as above), for callers that keep
+ * the result on the device.  No counterpart in the reference. */
+enum { LQCD_FIELD_CLOVER = 0, LQCD_FIELD_F_MU_NU = 1, LQCD_FIELD_MAGNETIC = 2 };
+int lqcd_field_strength_device(lqcd_lattice* lat, int what, double* d_out, size_t n);
+
 /* LatticeStateDefault::normalize_link_matrices (src/simulation/state.rs:754-756) */
 int lqcd_normalize_link_matrices(lqcd_lattice* lat);
 
@@ -329,7 +348,8 @@ typedef enum {
     LQCD_KERNEL_REUNIT = 5,
     LQCD_KERNEL_METROPOLIS = 6,
     LQCD_KERNEL_MOMENTA = 7,
-    LQCD_KERNEL_FORCE = 8
+    LQCD_KERNEL_FORCE = 8,
+    LQCD_KERNEL_FIELDS = 9
 } lqcd_kernel_kind;
 
 /* Copies `bytes` (a multiple of 16) between two device buffers `iters` times

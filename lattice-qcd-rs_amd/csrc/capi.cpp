@@ -456,6 +456,45 @@ int lqcd_efield_gauss(lqcd_lattice* L, double* out, size_t n_sites)
     return LQCD_OK;
 }
 
+static int clover_fields(lqcd_lattice* L, int mode, double* out, size_t n)
+{
+    CHECK_ARG(L && out, "null argument");
+    const size_t D = (size_t)L->g.dim;
+    const size_t want = (size_t)L->g.nsite * (mode == 2 ? D : D * D) * 18;
+    if (n != want)
+        return fail(LQCD_ERR_INCOMPATIBLE_SIZE, "size of lattice and data are incompatible: expected " +
+                                                    std::to_string(want) + " doubles, got " + std::to_string(n));
+    DeviceGuard dg(L->ctx->device);
+    DevBuf buf;
+    HIP_TRY(hipMalloc(&buf.p, sizeof(double) * want));
+    HIP_TRY(launch_clover_fields(L, mode, (double*)buf.p));
+    HIP_TRY(hipMemcpyAsync(out, buf.p, sizeof(double) * want, hipMemcpyDeviceToHost, L->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    return LQCD_OK;
+}
+
+int lqcd_clover_field(lqcd_lattice* L, double* out, size_t n) { return clover_fields(L, 0, out, n); }
+
+int lqcd_field_strength_device(lqcd_lattice* L, int what, double* d_out, size_t n)
+{
+    CHECK_ARG(L && d_out, "null argument");
+    CHECK_ARG(what == LQCD_FIELD_CLOVER || what == LQCD_FIELD_F_MU_NU || what == LQCD_FIELD_MAGNETIC,
+              "unknown field kind");
+    const size_t D = (size_t)L->g.dim;
+    const size_t want = (size_t)L->g.nsite * (what == LQCD_FIELD_MAGNETIC ? D : D * D) * 18;
+    if (n != want)
+        return fail(LQCD_ERR_INCOMPATIBLE_SIZE, "size of lattice and data are incompatible: expected " +
+                                                    std::to_string(want) + " doubles, got " + std::to_string(n));
+    DeviceGuard dg(L->ctx->device);
+    HIP_TRY(launch_clover_fields(L, what, d_out));
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    return LQCD_OK;
+}
+
+int lqcd_f_mu_nu_field(lqcd_lattice* L, double* out, size_t n) { return clover_fields(L, 1, out, n); }
+
+int lqcd_magnetic_field_vec(lqcd_lattice* L, double* out, size_t n) { return clover_fields(L, 2, out, n); }
+
 int lqcd_efield_gauss_sum_div(lqcd_lattice* L, double* out)
 {
     CHECK_ARG(L && out, "null argument");

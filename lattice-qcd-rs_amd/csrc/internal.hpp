@@ -65,7 +65,8 @@ enum ProfileKind {
     PROF_METROPOLIS = 6,
     PROF_MOMENTA = 7,
     PROF_FORCE_STORE = 8,
-    PROF_NKINDS = 9
+    PROF_FIELDS = 9,
+    PROF_NKINDS = 10
 };
 
 // ---- fields (k_fields.hip) ----
@@ -120,6 +121,10 @@ hipError_t launch_heat_bath_class(const lqcd_lattice* L, void* U, int mu, int pa
                                   uint64_t counter);
 // (reverse: OverrelaxationSweepReverse, else OverrelaxationSweepRotation)
 hipError_t launch_overrelax_class(const lqcd_lattice* L, void* U, int mu, int parity, bool reverse);
+
+// clover (mode 0), f_mu_nu (1) or magnetic_field_vec (2) of U[cur] at every
+// site, written to d_out in the host layout (k_clover.hip)
+hipError_t launch_clover_fields(const lqcd_lattice* L, int mode, double* d_out);
 
 // ---- Gauss law (k_gauss.hip); links are U[cur], G goes to gauss_g ----
 // G(x) of the double E copy Ed, and div = sum_x |Tr(sum_a T^a G(x))| into d_div

@@ -1,0 +1,246 @@
+// k_clover.hip -- clover, field-strength tensor and chromomagnetic field of
+// every site (LinkMatrix::clover / f_mu_nu / magnetic_field_vec,
+// src/field.rs:806-868), in double arithmetic on either storage type.
+//
+// One thread per site.  The output is written straight in the host layout of
+// the reference (18 doubles per matrix, column-major 3x3 complex):
+// [site][i][j] for the clover and F, [site][dir] for B.
+#include "dispatch.hpp"
+#include "fields.hpp"
+
+namespace lqcd {
+
+// neighbour of s one step along +d or -d (periodic)
+__device__ __forceinline__ int64_t site_step(const Geom& g, int64_t s, int d, bool pos)
+{
+    const int64_t L = g.extent[d], st = g.stride[d];
+    const int64_t x = (s / st) % L;
+    if (pos) return x == L - 1 ? s - (L - 1) * st : s + st;
+    return x == 0 ? s + (L - 1) * st : s - st;
+}
+
+__device__ __forceinline__ void adjoint(const mat3<double>& a, mat3<double>& b)
+{
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) b.e[i * 3 + j] = {a.e[j * 3 + i].re, -a.e[j * 3 + i].im};
+}
+
+// LinkMatrix::matrix (field.rs:726-740): U_d(y) for +d, U_d(y - d)^+ for -d
+template <typename S>
+__device__ __forceinline__ mat3<double> link_signed(const vec2_t<S>* __restrict__ U, const Geom& g, int64_t y,
+                                                   int d, bool pos)
+{
+    if (pos) return load_link<double, S>(U, g.npad, d, y);
+    const mat3<double> m = load_link<double, S>(U, g.npad, d, site_step(g, y, d, false));
+    mat3<double> r;
+    adjoint(m, r);
+    return r;
+}
+
+// LinkMatrix::pij (field.rs:743-771) for signed directions a, b:
+//   S_ab(x) = U_b(x) U_a(x+b) U_b(x+a)^+,  P_ab(x) = U_a(x) S_ab(x)^+
+template <typename S>
+__device__ __forceinline__ void pij_signed(const vec2_t<S>* __restrict__ U, const Geom& g, int64_t s, int a,
+                                           bool pa, int b, bool pb, mat3<double>& p)
+{
+    const int64_t xb = site_step(g, s, b, pb), xa = site_step(g, s, a, pa);
+    mat3<double> t, sab;
+    {
+        const mat3<double> ub = link_signed<S>(U, g, s, b, pb);
+        const mat3<double> ua_b = link_signed<S>(U, g, xb, a, pa);
+        mul<false, false>(ub, ua_b, t);
+    }
+    {
+        const mat3<double> ub_a = link_signed<S>(U, g, xa, b, pb);
+        mul<false, true>(t, ub_a, sab);
+    }
+    const mat3<double> ua = link_signed<S>(U, g, s, a, pa);
+    mul<false, true>(ua, sab, p);
+}
+
+// LinkMatrix::clover (field.rs:806-820):
+//   P(i, j) + P(j, -i) + P(-i, -j) + P(-j, i)
+template <typename S>
+__device__ __forceinline__ void clover_at(const vec2_t<S>* __restrict__ U, const Geom& g, int64_t s, int i,
+                                          int j, mat3<double>& c)
+{
+    mat3<double> p;
+    pij_signed<S>(U, g, s, i, true, j, true, c);
+    pij_signed<S>(U, g, s, j, true, i, false, p);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) c.e[k] = {c.e[k].re + p.e[k].re, c.e[k].im + p.e[k].im};
+    pij_signed<S>(U, g, s, i, false, j, false, p);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) c.e[k] = {c.e[k].re + p.e[k].re, c.e[k].im + p.e[k].im};
+    pij_signed<S>(U, g, s, j, false, i, true, p);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) c.e[k] = {c.e[k].re + p.e[k].re, c.e[k].im + p.e[k].im};
+}
+
+// One leaf of the clover in the form of pij: S = op(ub) op(uab) op(uba)^+,
+// P = op(ua) S^+, where op(.) is the adjoint when the template flag is set
+// and each matrix is the link U_d(site) named by its arguments.
+template <typename S, bool DUB, bool DUAB, bool DUBA, bool DUA>
+__device__ __forceinline__ void leaf(const vec2_t<S>* __restrict__ U, const Geom& g, int db, int64_t sb,
+                                     int dab, int64_t sab, int dba, int64_t sba, int da, int64_t sa,
+                                     mat3<double>& p)
+{
+    mat3<double> t, st;
+    {
+        const mat3<double> ub = load_link<double, S>(U, g.npad, db, sb);
+        const mat3<double> uab = load_link<double, S>(U, g.npad, dab, sab);
+        mul<DUB, DUAB>(ub, uab, t);
+    }
+    {
+        const mat3<double> uba = load_link<double, S>(U, g.npad, dba, sba);
+        mul<false, !DUBA>(t, uba, st);
+    }
+    const mat3<double> ua = load_link<double, S>(U, g.npad, da, sa);
+    mul<DUA, true>(ua, st, p);
+}
+
+// clover for i != j with the site's neighbour offsets: an offset along one
+// direction depends only on that coordinate, so offsets along different
+// directions add.  The four leaves are pij(i, j), pij(j, -i), pij(-i, -j),
+// pij(-j, i) with U_{-d}(y) = U_d(y - d)^+ (field.rs:726-771):
+//   P(i,j):   ub = U_j(x),      uab = U_i(x+j),      uba = U_j(x+i),      ua = U_i(x)
+//   P(j,-i):  ub = U_i(x-i)^+,  uab = U_j(x-i),      uba = U_i(x+j-i)^+,  ua = U_j(x)
+//   P(-i,-j): ub = U_j(x-j)^+,  uab = U_i(x-j-i)^+,  uba = U_j(x-i-j)^+,  ua = U_i(x-i)^+
+//   P(-j,i):  ub = U_i(x),      uab = U_j(x+i-j)^+,  uba = U_i(x-j),      ua = U_j(x-j)^+
+template <int D, typename S>
+__device__ __forceinline__ void clover_offdiag(const vec2_t<S>* __restrict__ U, const Geom& g, int64_t s,
+                                               const Neighbours<D>& nb, int i, int j, mat3<double>& c)
+{
+    const int64_t pi = s + nb.up[i], pj = s + nb.up[j], mi = s + nb.dn[i], mj = s + nb.dn[j];
+    const int64_t pj_mi = pj + nb.dn[i], mi_mj = mi + nb.dn[j], pi_mj = pi + nb.dn[j];
+    // one leaf at a time: without the scheduling fences the loads of all
+    // four leaves are issued up front and the kernel needs 250 VGPRs
+    mat3<double> p;
+    leaf<S, false, false, false, false>(U, g, j, s, i, pj, j, pi, i, s, c);
+    __builtin_amdgcn_sched_barrier(0);
+    leaf<S, true, false, true, false>(U, g, i, mi, j, mi, i, pj_mi, j, s, p);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) c.e[k] = {c.e[k].re + p.e[k].re, c.e[k].im + p.e[k].im};
+    __builtin_amdgcn_sched_barrier(0);
+    leaf<S, true, true, true, true>(U, g, j, mj, i, mi_mj, j, mi_mj, i, mi, p);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) c.e[k] = {c.e[k].re + p.e[k].re, c.e[k].im + p.e[k].im};
+    __builtin_amdgcn_sched_barrier(0);
+    leaf<S, false, true, false, true>(U, g, i, s, j, pi_mj, i, mj, j, mj, p);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) c.e[k] = {c.e[k].re + p.e[k].re, c.e[k].im + p.e[k].im};
+}
+
+// 18 doubles, column-major, at out
+__device__ __forceinline__ void store_host_matrix(double* __restrict__ out, const mat3<double>& m)
+{
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+            *reinterpret_cast<double2*>(out + 2 * (c * 3 + r)) = make_double2(m.e[r * 3 + c].re, m.e[r * 3 + c].im);
+}
+
+__device__ __forceinline__ void add_host_matrix(double* __restrict__ out, const mat3<double>& m)
+{
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            double2* q = reinterpret_cast<double2*>(out + 2 * (c * 3 + r));
+            const double2 v = *q;
+            *q = make_double2(v.x + m.e[r * 3 + c].re, v.y + m.e[r * 3 + c].im);
+        }
+}
+
+// levi_civita(d, i, j) for i < j and d outside {i, j} (src/utils.rs:306-318):
+// sign(i - d) * sign(j - d)
+__device__ __forceinline__ double levi_dij(int d, int i, int j)
+{
+    return (i > d) == (j > d) ? 1.0 : -1.0;
+}
+
+// MODE 0: clover; 1: f_mu_nu; 2: magnetic_field_vec.  inv = 1 / (8 a^2).
+//
+// Pairs i < j are computed once: clover(j, i) = clover(i, j)^+ (each leaf of
+// one is the adjoint of a leaf of the other), so F_ji = -F_ij and
+// F_ij = (C - C^+) / (8 a^2).  B_d = sum_{i,j} eps(d,i,j) F_ij / (2i) takes
+// each unordered pair twice with the same sign, i.e. eps(d,i,j) F_ij / i.
+template <int D, typename S, int MODE>
+__global__ __launch_bounds__(kBlock) void k_clover_fields(const vec2_t<S>* __restrict__ U, Geom g,
+                                                          double* __restrict__ out, double inv)
+{
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= g.nsite) return;
+    const Neighbours<D> nb = neighbours<D>(g, s);
+    const mat3<double> zero = {};
+    if (MODE == 2) {
+        for (int d = 0; d < D; ++d) store_host_matrix(out + 18 * (s * D + d), zero);
+    }
+    for (int i = 0; i < D; ++i) {
+        if (MODE == 0) {
+            mat3<double> c;
+            clover_at<S>(U, g, s, i, i, c);
+            store_host_matrix(out + 18 * ((s * D + i) * D + i), c);
+        } else if (MODE == 1) {
+            store_host_matrix(out + 18 * ((s * D + i) * D + i), zero);
+        }
+        for (int j = i + 1; j < D; ++j) {
+            mat3<double> c, ca;
+            clover_offdiag<D, S>(U, g, s, nb, i, j, c);
+            adjoint(c, ca);
+            if (MODE == 0) {
+                store_host_matrix(out + 18 * ((s * D + i) * D + j), c);
+                store_host_matrix(out + 18 * ((s * D + j) * D + i), ca);
+                continue;
+            }
+            mat3<double> f, fm;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                f.e[k] = {(c.e[k].re - ca.e[k].re) * inv, (c.e[k].im - ca.e[k].im) * inv};
+                fm.e[k] = {-f.e[k].re, -f.e[k].im};
+            }
+            if (MODE == 1) {
+                store_host_matrix(out + 18 * ((s * D + i) * D + j), f);
+                store_host_matrix(out + 18 * ((s * D + j) * D + i), fm);
+                continue;
+            }
+            for (int d = 0; d < D; ++d) {
+                if (d == i || d == j) continue;
+                const double sg = levi_dij(d, i, j);
+                mat3<double> b;   // sg * F / i = sg * (-i F)
+#pragma unroll
+                for (int k = 0; k < 9; ++k) b.e[k] = {sg * f.e[k].im, -sg * f.e[k].re};
+                add_host_matrix(out + 18 * (s * D + d), b);
+            }
+        }
+    }
+}
+
+hipError_t launch_clover_fields(const lqcd_lattice* L, int mode, double* d_out)
+{
+    const unsigned grid = grid_for(L->g.nsite);
+    const double inv = 1.0 / (8.0 * L->size * L->size);
+    lqcd_prof_event ev;
+    prof_begin(L->ctx, PROF_FIELDS, &ev);
+    switch (mode) {
+    case 0:
+        LQCD_DISPATCH_DS(L, k_clover_fields<D, S, 0><<<grid, kBlock, 0, L->ctx->stream>>>(
+                                (const vec2_t<S>*)L->U[L->cur], L->g, d_out, inv));
+        break;
+    case 1:
+        LQCD_DISPATCH_DS(L, k_clover_fields<D, S, 1><<<grid, kBlock, 0, L->ctx->stream>>>(
+                                (const vec2_t<S>*)L->U[L->cur], L->g, d_out, inv));
+        break;
+    default:
+        LQCD_DISPATCH_DS(L, k_clover_fields<D, S, 2><<<grid, kBlock, 0, L->ctx->stream>>>(
+                                (const vec2_t<S>*)L->U[L->cur], L->g, d_out, inv));
+        break;
+    }
+    prof_end(L->ctx, &ev);
+    return hipGetLastError();
+}
+
+} // namespace lqcd

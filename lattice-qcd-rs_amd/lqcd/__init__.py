@@ -148,6 +148,17 @@ class LatticeCyclic:
     def number_of_canonical_links_space(self) -> int:
         return self.number_of_points() * self._d
 
+    def index_of(self, point) -> int:
+        """Site index sum_i (x_i mod L_i) prod_{k<i} L_k (``to_index``,
+        src/lattice.rs:909-929), coordinates taken modulo the extents."""
+        if len(point) != self._d:
+            raise ValueError("one coordinate per direction")
+        idx, stride = 0, 1
+        for x, n in zip(point, self._extent):
+            idx += (int(x) % n) * stride
+            stride *= n
+        return idx
+
 
 class _DeviceLattice:
     """Owner of one lqcd_lattice handle (links, E field, work buffers)."""
@@ -240,6 +251,54 @@ class LatticeStateDefault:
 
     def normalize_link_matrices(self) -> None:
         _raise(F.lib().lqcd_normalize_link_matrices(self.handle))
+
+    # clover / F_mu_nu / chromomagnetic field (LinkMatrix, src/field.rs:806-882),
+    # computed on the device for every site; matrices as 18 doubles
+    # (column-major 3x3 complex), like the links
+    def clover_field(self) -> np.ndarray:
+        """``clover(x, i, j)`` for every site and ordered pair, ``(nsite, D, D, 18)``."""
+        return self._fields(F.lib().lqcd_clover_field, True)
+
+    def f_mu_nu_field(self) -> np.ndarray:
+        """``f_mu_nu(x, i, j)`` for every site and ordered pair, ``(nsite, D, D, 18)``."""
+        return self._fields(F.lib().lqcd_f_mu_nu_field, True)
+
+    def magnetic_field_vec_field(self) -> np.ndarray:
+        """``magnetic_field_vec(x)`` for every site, ``(nsite, D, 18)``."""
+        return self._fields(F.lib().lqcd_magnetic_field_vec, False)
+
+    def _fields(self, fn, pairs: bool) -> np.ndarray:
+        lat = self.lattice()
+        d = lat.dim_st()
+        shape = (lat.number_of_points(), d, d, 18) if pairs else (lat.number_of_points(), d, 18)
+        out = np.empty(shape, dtype=np.float64)
+        _raise(fn(self.handle, F.ptr(out), out.size))
+        return out
+
+    def _site(self, point) -> int:
+        return self.lattice().index_of([int(c) for c in point])
+
+    def clover(self, point, dir_i: int, dir_j: int) -> np.ndarray:
+        """``LinkMatrix::clover`` (src/field.rs:806-820) at one point (coordinates
+        taken modulo the extents), positive directions."""
+        return self.clover_field()[self._site(point), dir_i, dir_j]
+
+    def f_mu_nu(self, point, dir_i: int, dir_j: int) -> np.ndarray:
+        """``LinkMatrix::f_mu_nu`` (src/field.rs:822-834) at one point."""
+        return self.f_mu_nu_field()[self._site(point), dir_i, dir_j]
+
+    def magnetic_field_vec(self, point) -> np.ndarray:
+        """``LinkMatrix::magnetic_field_vec`` (src/field.rs:836-845) at one point, ``(D, 18)``."""
+        return self.magnetic_field_vec_field()[self._site(point)]
+
+    def magnetic_field(self, point, dir: int) -> np.ndarray:
+        """``LinkMatrix::magnetic_field`` (src/field.rs:848-868) at one point."""
+        return self.magnetic_field_vec(point)[dir]
+
+    def magnetic_field_link(self, point, dir: int) -> np.ndarray:
+        """``LinkMatrix::magnetic_field_link`` (src/field.rs:871-878): the field at
+        the link's position along its direction."""
+        return self.magnetic_field(point, dir)
 
     # MD work decomposition (0 per link, 1 per link XCD-grouped, 2 per site,
     # 3 per site with slab order)

@@ -39,6 +39,8 @@ KERNEL_REUNIT = 5
 KERNEL_METROPOLIS = 6
 KERNEL_MOMENTA = 7
 KERNEL_FORCE = 8
+KERNEL_FIELDS = 9
+FIELD_CLOVER, FIELD_F_MU_NU, FIELD_MAGNETIC = 0, 1, 2
 
 
 class HmcStats(C.Structure):
@@ -126,6 +128,10 @@ SIGNATURES = {
     "lqcd_metropolis_sweep_random": (_I, [_P, _D, _S, _D, _U64, _U64, C.POINTER(MhStats)]),
     "lqcd_heat_bath_sweep": (_I, [_P, _D, _U64, _U64, C.POINTER(C.c_size_t)]),
     "lqcd_overrelaxation_sweep": (_I, [_P]),
+    "lqcd_clover_field": (_I, [_P, _P, _S]),
+    "lqcd_f_mu_nu_field": (_I, [_P, _P, _S]),
+    "lqcd_magnetic_field_vec": (_I, [_P, _P, _S]),
+    "lqcd_field_strength_device": (_I, [_P, _I, _P, _S]),
     "lqcd_overrelaxation_reverse_sweep": (_I, [_P]),
     "lqcd_normalize_link_matrices": (_I, [_P]),
     "lqcd_stream_copy": (_I, [_P, _S, _I, C.POINTER(_D)]),
@@ -189,3 +195,62 @@ def ptr(a: np.ndarray):
     if not a.flags["C_CONTIGUOUS"]:
         raise ValueError("array must be C-contiguous")
     return a.ctypes.data_as(C.c_void_p)
+
+
+_hip = None
+
+
+def hip_runtime():
+    """The HIP runtime liblqcd_hip.so links against (loaded with it).  Device
+    buffers handed to the library's device-pointer entry points must come from
+    this runtime: a second HIP runtime in the process (a framework's bundled
+    one) does not see the library's device state."""
+    global _hip
+    if _hip is None:
+        lib()
+        h = C.CDLL("libamdhip64.so.7", mode=C.RTLD_GLOBAL)
+        h.hipMalloc.restype = C.c_int
+        h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        h.hipFree.restype = C.c_int
+        h.hipFree.argtypes = [C.c_void_p]
+        h.hipMemcpy.restype = C.c_int
+        h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        h.hipDeviceSynchronize.restype = C.c_int
+        h.hipDeviceSynchronize.argtypes = []
+        _hip = h
+    return _hip
+
+
+class DeviceBuffer:
+    """``nbytes`` of device memory on the current device, freed with the object."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        self._p = C.c_void_p()
+        rc = hip_runtime().hipMalloc(C.byref(self._p), self.nbytes)
+        if rc != 0:
+            raise LqcdError(ERR_OUT_OF_MEMORY, f"hipMalloc({self.nbytes}) failed with {rc}")
+
+    @property
+    def ptr(self) -> int:
+        return int(self._p.value)
+
+    def to_numpy(self, dtype=np.float64) -> np.ndarray:
+        out = np.empty(self.nbytes // np.dtype(dtype).itemsize, dtype=dtype)
+        rc = hip_runtime().hipMemcpy(out.ctypes.data_as(C.c_void_p), self._p, out.nbytes, 2)  # D2H
+        if rc != 0:
+            raise LqcdError(ERR_DEVICE, f"hipMemcpy failed with {rc}")
+        return out
+
+    def fill_from(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a)
+        if a.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        rc = hip_runtime().hipMemcpy(self._p, a.ctypes.data_as(C.c_void_p), a.nbytes, 1)  # H2D
+        if rc != 0:
+            raise LqcdError(ERR_DEVICE, f"hipMemcpy failed with {rc}")
+
+    def __del__(self):
+        if _hip is not None and self._p:
+            _hip.hipFree(self._p)
+            self._p = C.c_void_p()

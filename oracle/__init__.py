@@ -133,6 +133,12 @@ def _declare(L):
         "or_sij": (None, [LP, _P, _S, _I, _I, _I, _I, _P]),
         "or_pij": (None, [LP, _P, _S, _I, _I, _I, _I, _P]),
         "or_clover": (None, [LP, _P, _S, _I, _I, _P]),
+        "or_f_mu_nu": (None, [LP, _P, _S, _I, _I, _P]),
+        "or_levi_civita": (_I, [_P, _I]),
+        "or_magnetic_field": (None, [LP, _P, _S, _I, _P]),
+        "or_clover_field": (None, [LP, _P, _P]),
+        "or_f_mu_nu_field": (None, [LP, _P, _P]),
+        "or_magnetic_field_vec": (None, [LP, _P, _P]),
         "or_average_trace_plaquette": (Cplx, [LP, _P]),
         "or_hamiltonian_links": (_D, [LP, _P, _D]),
         "or_hamiltonian_efield": (_D, [LP, _P, _D]),
@@ -286,6 +292,34 @@ class OracleLattice:
     def clover(self, u, s, di, dj):
         out = np.zeros(18)
         lib().or_clover(self.ref, _ptr(u), s, di, dj, _ptr(out))
+        return out
+
+    def f_mu_nu(self, u, s, di, dj):
+        out = np.zeros(18)
+        lib().or_f_mu_nu(self.ref, _ptr(u), s, di, dj, _ptr(out))
+        return out
+
+    def magnetic_field(self, u, s, d):
+        out = np.zeros(18)
+        lib().or_magnetic_field(self.ref, _ptr(u), s, d, _ptr(out))
+        return out
+
+    def clover_field(self, u):
+        """[site][i][j] clovers, (nsite, D, D, 18)"""
+        out = np.zeros((self.nsite, self.dim, self.dim, 18))
+        lib().or_clover_field(self.ref, _ptr(u), _ptr(out))
+        return out
+
+    def f_mu_nu_field(self, u):
+        """[site][i][j] F_ij, (nsite, D, D, 18)"""
+        out = np.zeros((self.nsite, self.dim, self.dim, 18))
+        lib().or_f_mu_nu_field(self.ref, _ptr(u), _ptr(out))
+        return out
+
+    def magnetic_field_vec(self, u):
+        """[site][dir] B_dir, (nsite, D, 18)"""
+        out = np.zeros((self.nsite, self.dim, 18))
+        lib().or_magnetic_field_vec(self.ref, _ptr(u), _ptr(out))
         return out
 
     def average_trace_plaquette(self, u) -> complex:
@@ -482,6 +516,11 @@ def polar_unitary(m18) -> np.ndarray:
     out = np.zeros(18)
     lib().or_polar_unitary(_ptr(m), _ptr(out))
     return out
+
+
+def levi_civita(index) -> int:
+    idx = np.ascontiguousarray(index, dtype=np.int32)
+    return int(lib().or_levi_civita(_ptr(idx), len(idx)))
 
 
 def svd3(m18):

@@ -869,6 +869,99 @@ void or_clover(const or_lattice* l, const double* links, size_t site, int di, in
     madd(&s, &d, out);
 }
 
+/* num-complex division (a + bi) / (c + di) = ((ac + bd) + (bc - ad) i) / (c^2 + d^2),
+ * as used for matrix / Complex in src/field.rs:833,867 */
+static or_cplx cdiv_nc(or_cplx x, or_cplx y)
+{
+    const double n = y.re * y.re + y.im * y.im;
+    return cx((x.re * y.re + x.im * y.im) / n, (x.im * y.re - x.re * y.im) / n);
+}
+
+/* LinkMatrix::f_mu_nu, src/field.rs:822-834:
+ * (clover(i, j) - clover(j, i)) / (8 a^2) */
+void or_f_mu_nu(const or_lattice* l, const double* links, size_t site, int di, int dj, or_mat3* out)
+{
+    or_mat3 a, b, m;
+    or_clover(l, links, site, di, dj, &a);
+    or_clover(l, links, site, dj, di, &b);
+    msub(&a, &b, &m);
+    const or_cplx den = cx(8.0 * l->size * l->size, 0.0);
+    for (int k = 0; k < 9; ++k) out->m[k] = cdiv_nc(m.m[k], den);
+}
+
+/* levi_civita, src/utils.rs:306-318: product over pairs j < i of
+ * sign(index[i] - index[j]) */
+int or_levi_civita(const int* index, int n)
+{
+    int prod = 1;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < i; ++j) prod *= index[i] == index[j] ? 0 : (index[i] > index[j] ? 1 : -1);
+    return prod;
+}
+
+/* LinkMatrix::magnetic_field, src/field.rs:848-868:
+ * sum_{i, j} f_mu_nu(i, j) * levi_civita(dir, i, j), divided by 2i */
+void or_magnetic_field(const or_lattice* l, const double* links, size_t site, int dir, or_mat3* out)
+{
+    or_mat3 sum;
+    mzero(&sum);
+    for (int i = 0; i < l->dim; ++i) {
+        or_mat3 inner;
+        mzero(&inner);
+        for (int j = 0; j < l->dim; ++j) {
+            or_mat3 f;
+            or_f_mu_nu(l, links, site, i, j, &f);
+            const int idx[3] = {dir, i, j};
+            const double lc = (double)or_levi_civita(idx, 3);
+            for (int k = 0; k < 9; ++k) f.m[k] = cmul(f.m[k], cx(lc, 0.0));
+            madd(&inner, &f, &inner);
+        }
+        madd(&sum, &inner, &sum);
+    }
+    for (int k = 0; k < 9; ++k) out->m[k] = cdiv_nc(sum.m[k], cx(0.0, 2.0));
+}
+
+/* the three observables over the whole lattice, in the host layouts
+ * [site][i][j] (clover, f_mu_nu) and [site][dir] (magnetic_field_vec,
+ * src/field.rs:836-845), 18 doubles per matrix */
+void or_clover_field(const or_lattice* l, const double* links, double* out)
+{
+    const size_t D = (size_t)l->dim;
+#pragma omp parallel for OMP_THREADS schedule(static)
+    for (size_t s = 0; s < l->nsite; ++s)
+        for (int i = 0; i < l->dim; ++i)
+            for (int j = 0; j < l->dim; ++j) {
+                or_mat3 m;
+                or_clover(l, links, s, i, j, &m);
+                store_mat(out + 18 * ((s * D + (size_t)i) * D + (size_t)j), &m);
+            }
+}
+
+void or_f_mu_nu_field(const or_lattice* l, const double* links, double* out)
+{
+    const size_t D = (size_t)l->dim;
+#pragma omp parallel for OMP_THREADS schedule(static)
+    for (size_t s = 0; s < l->nsite; ++s)
+        for (int i = 0; i < l->dim; ++i)
+            for (int j = 0; j < l->dim; ++j) {
+                or_mat3 m;
+                or_f_mu_nu(l, links, s, i, j, &m);
+                store_mat(out + 18 * ((s * D + (size_t)i) * D + (size_t)j), &m);
+            }
+}
+
+void or_magnetic_field_vec(const or_lattice* l, const double* links, double* out)
+{
+    const size_t D = (size_t)l->dim;
+#pragma omp parallel for OMP_THREADS schedule(static)
+    for (size_t s = 0; s < l->nsite; ++s)
+        for (int d = 0; d < l->dim; ++d) {
+            or_mat3 m;
+            or_magnetic_field(l, links, s, d, &m);
+            store_mat(out + 18 * (s * D + (size_t)d), &m);
+        }
+}
+
 /* per-site sum of Tr P_ij over i < j, in the nested order of
  * LinkMatrix::average_trace_plaquette, src/field.rs:788-801 */
 static or_cplx site_trace_plaquette(const or_lattice* l, const double* links, size_t site)

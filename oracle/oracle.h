@@ -100,6 +100,12 @@ void    or_pij(const or_lattice* l, const double* links, size_t site, int di, in
                int dj, int pj, or_mat3* out);
 void    or_clover(const or_lattice* l, const double* links, size_t site, int di, int dj,
                   or_mat3* out);
+void    or_f_mu_nu(const or_lattice* l, const double* links, size_t site, int di, int dj, or_mat3* out);
+int     or_levi_civita(const int* index, int n);
+void    or_magnetic_field(const or_lattice* l, const double* links, size_t site, int dir, or_mat3* out);
+void    or_clover_field(const or_lattice* l, const double* links, double* out);
+void    or_f_mu_nu_field(const or_lattice* l, const double* links, double* out);
+void    or_magnetic_field_vec(const or_lattice* l, const double* links, double* out);
 or_cplx or_average_trace_plaquette(const or_lattice* l, const double* links);
 double  or_hamiltonian_links(const or_lattice* l, const double* links, double beta);
 double  or_hamiltonian_efield(const or_lattice* l, const double* e, double beta);

@@ -84,3 +84,30 @@ def expi(v: np.ndarray) -> np.ndarray:
     h = su3_matrix(v)
     w, q = np.linalg.eigh(h)
     return (q * np.exp(1j * w)[..., None, :]) @ dag(q)
+
+
+def clover_field(f: np.ndarray, i: int, j: int) -> np.ndarray:
+    """The four (i, j) plaquettes around x, each a closed loop from x:
+    C_ij(x) = U_i(x) U_j(x+i) U_i(x+j)^+ U_j(x)^+
+            + U_j(x) U_i(x+j-i)^+ U_j(x-i)^+ U_i(x-i)
+            + U_i(x-i)^+ U_j(x-i-j)^+ U_i(x-i-j) U_j(x-j)
+            + U_j(x-j)^+ U_i(x-j) U_j(x-j+i) U_i(x)^+"""
+    dim = f.shape[-3]
+    ui = f[..., i, :, :]
+    uj = f[..., j, :, :]
+    ui_mi = shift(ui, i, dim, -1)
+    uj_mj = shift(uj, j, dim, -1)
+    q1 = ui @ shift(uj, i, dim) @ dag(shift(ui, j, dim)) @ dag(uj)
+    q2 = uj @ dag(shift(ui_mi, j, dim)) @ dag(shift(uj, i, dim, -1)) @ ui_mi
+    q3 = dag(ui_mi) @ dag(shift(uj_mj, i, dim, -1)) @ shift(ui_mi, j, dim, -1) @ uj_mj
+    q4 = dag(uj_mj) @ shift(ui, j, dim, -1) @ shift(uj_mj, i, dim) @ dag(ui)
+    return q1 + q2 + q3 + q4
+
+
+def permutation_sign(idx) -> int:
+    """Sign of the permutation that sorts idx (0 when a value repeats)."""
+    idx = list(idx)
+    if len(set(idx)) != len(idx):
+        return 0
+    inv = sum(1 for a in range(len(idx)) for b in range(a + 1, len(idx)) if idx[a] > idx[b])
+    return -1 if inv % 2 else 1

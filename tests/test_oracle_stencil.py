@@ -6,8 +6,8 @@ import numpy as np
 import pytest
 
 import oracle as O
-from _lattice_np import (T, dag, expi, hamiltonian_links_np, links_to_field, field_to_links,
-                         staple_field, su3_matrix)
+from _lattice_np import (T, clover_field, dag, expi, hamiltonian_links_np, links_to_field,
+                         field_to_links, permutation_sign, staple_field, su3_matrix)
 
 EPSILON = 1e-9
 
@@ -37,6 +37,102 @@ def test_clover_known_answers():
     # direct pij consequences: P_xy(0) = i I when U_x(0) = i I
     assert np.allclose(mat(l.pij(u2, 0, x, 1, y, 1)), 1j * np.eye(3), atol=1e-15)
     assert np.allclose(mat(l.pij(u2, 0, y, 1, x, 1)), -1j * np.eye(3), atol=1e-15)
+
+
+def test_magnetic_field_known_answers():
+    # the rest of src/field.rs:1579-1711 (test magnetic_field, D=3, L=4)
+    l = O.OracleLattice(3, 4)
+    x, y, z = 0, 1, 2
+    zero = np.zeros((3, 3))
+    u = l.cold_start()
+    assert np.linalg.norm(mat(l.f_mu_nu(u, 0, x, y)) - zero) < EPSILON
+    assert np.linalg.norm(mat(l.magnetic_field(u, 0, x)) - zero) < EPSILON
+    assert np.all(np.abs(l.magnetic_field_vec(u)) < EPSILON)
+    u2 = u.copy()
+    u2[0] = O.from_np(1j * np.eye(3))
+    assert np.linalg.norm(mat(l.f_mu_nu(u2, 0, x, y))) < EPSILON
+    for d in range(3):
+        assert np.linalg.norm(mat(l.magnetic_field(u2, 0, d))) < EPSILON
+    u3 = u.copy()
+    u3[l.site_index([1, 0, 0]) * 3 + y] = O.from_np(1j * np.eye(3))
+    assert np.linalg.norm(mat(l.f_mu_nu(u3, 0, x, y)) - 0.25j * np.eye(3)) < EPSILON
+    assert np.linalg.norm(mat(l.magnetic_field(u3, 0, x))) < EPSILON
+    b = mat(l.magnetic_field(u3, 0, z))
+    assert np.linalg.norm(b - 0.25 * np.eye(3)) < EPSILON
+    bv = l.magnetic_field_vec(u3)
+    s0 = l.site_index([0, 0, 0])
+    for d in range(3):
+        want = b if d == 2 else zero
+        assert np.linalg.norm(mat(bv[s0, d]) - want) < EPSILON
+    # [4, 0, 0] is [0, 0, 0] on the periodic lattice: the same site index
+    assert l.site_index([0, 0, 0]) == s0
+
+
+def test_levi_civita_known_values():
+    # src/utils.rs:360-372
+    for idx, want in [([], 1), ([1, 2], 1), ([0, 1], 1), ([1, 2, 3], 1), ([0, 1, 2], 1),
+                      ([3, 1, 2], 1), ([2, 3, 1], 1), ([3, 1, 2, 4], 1), ([1, 3, 4, 2], 1),
+                      ([3, 3, 1], 0), ([1, 1, 1], 0), ([1, 1], 0)]:
+        assert O.levi_civita(idx) == want, idx
+    for idx in [[1, 0], [0, 2, 1], [2, 1, 0], [0, 1, 3, 2], [3, 0, 1]]:
+        assert O.levi_civita(idx) == permutation_sign(idx), idx
+
+
+@pytest.mark.parametrize("dim,ext", [(4, [4, 3, 4, 2]), (3, [3, 4, 5]), (2, [4, 6])])
+def test_clover_f_b_against_numpy_shifts(dim, ext):
+    # clover (field.rs:806-820) as four closed loops from x written with array
+    # shifts; F = (C_ij - C_ji) / (8 a^2) (field.rs:822-834);
+    # B_d = sum_{i,j} eps(d, i, j) F_ij / (2i) (field.rs:848-868)
+    a = 0.7
+    l = O.OracleLattice(dim, ext, a)
+    u = l.hot_start(21)
+    f = links_to_field(u, ext)
+    cl = l.clover_field(u)
+    fm = l.f_mu_nu_field(u)
+    bv = l.magnetic_field_vec(u)
+    C = {}
+    for i in range(dim):
+        for j in range(dim):
+            C[i, j] = clover_field(f, i, j).reshape(-1, 3, 3)
+            assert np.max(np.abs(O.to_np(cl[:, i, j]) - C[i, j])) < 1e-12
+    F = {}
+    for i in range(dim):
+        for j in range(dim):
+            F[i, j] = (C[i, j] - C[j, i]) / (8 * a * a)
+            assert np.max(np.abs(O.to_np(fm[:, i, j]) - F[i, j])) < 1e-12
+    for d in range(dim):
+        b = sum(permutation_sign([d, i, j]) * F[i, j] for i in range(dim) for j in range(dim)) / 2j
+        assert np.max(np.abs(O.to_np(bv[:, d]) - b)) < 1e-12
+    # algebra: C_ji = C_ij^+, so F is antihermitian and antisymmetric and B hermitian
+    for i in range(dim):
+        for j in range(dim):
+            assert np.max(np.abs(C[j, i] - dag(C[i, j]))) < 1e-12
+            assert np.max(np.abs(O.to_np(fm[:, i, j]) + dag(O.to_np(fm[:, i, j])))) < 1e-12
+    for d in range(dim):
+        b = O.to_np(bv[:, d])
+        assert np.max(np.abs(b - dag(b))) < 1e-12
+    if dim == 2:
+        assert np.all(bv == 0.0)
+
+
+def test_clover_is_gauge_covariant():
+    # every leaf starts and ends at x: C_ij(x) -> g(x) C_ij(x) g(x)^+ under
+    # U_mu(x) -> g(x) U_mu(x) g(x+mu)^+; a wrong leaf orientation breaks this
+    ext = [4, 4, 2, 4]
+    l = O.OracleLattice(4, ext)
+    u = l.hot_start(5)
+    g = np.array([O.to_np(m) for m in l.hot_start(6).reshape(-1, 4, 18)[:, 0]])
+    f = links_to_field(u, ext).reshape(-1, 4, 3, 3)
+    nbr = np.array([[l.neighbor(s, mu, True) for mu in range(4)] for s in range(l.nsite)])
+    ft = np.empty_like(f)
+    for mu in range(4):
+        ft[:, mu] = g @ f[:, mu] @ dag(g[nbr[:, mu]])
+    ut = O.from_np(ft.reshape(-1, 3, 3)).reshape(-1, 18)
+    c0 = O.to_np(l.clover_field(u))
+    c1 = O.to_np(l.clover_field(ut))
+    for i in range(4):
+        for j in range(4):
+            assert np.max(np.abs(c1[:, i, j] - g @ c0[:, i, j] @ dag(g))) < 1e-12
 
 
 @pytest.mark.parametrize("dim,L", [(3, 4), (4, 2), (4, 3)])
