@@ -69,7 +69,7 @@ def parse():
     p.add_argument("--copy-only", action="store_true",
                    help="only run the 1 GiB streaming copy (byte-counter calibration)")
     p.add_argument("--workload", default="hmc32",
-                   choices=["hmc32", "batched", "large", "mh3d", "small", "fields"])
+                   choices=["hmc32", "batched", "large", "mh3d", "small", "fields", "state"])
     p.add_argument("--precision-trajectories", type=int, default=3,
                    help="trajectories of the FP32-mixed against FP64 error table (0 = skip)")
     p.add_argument("--gauss-checks", type=int, default=4000,
@@ -482,6 +482,38 @@ def bench_fields(args, lqcd):
         cfg, roofline_from(rate, "k_clover_fields<4,float,1> (clover, F_mu_nu)"), cpu)
 
 
+def bench_state(args, lqcd):
+    """LatticeStateEFSyncDefault written and read back in the serde bincode
+    layout (state.rs:1046-1062): device download + encode + file, and file +
+    decode + upload, on the 32^4 f32 lattice (files under the temp dir)."""
+    import tempfile
+    F = lqcd._ffi
+    L = args.extent
+    st = lqcd.LatticeStateDefault.new_determinist(1.0, args.beta, L, args.seed, 4, lqcd.F32)
+    ef = lqcd.LatticeStateEFSyncDefault.new_random_e_state(st, args.seed, 0, project=False)
+    nlink = st.lattice().number_of_canonical_links_space()
+    nbytes = 8 + nlink * 64 + 8 + 32 + nlink * 144
+    with tempfile.TemporaryDirectory() as d:
+        path = f"{d}/state.bin"
+        ef.write(path)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ef.write(path)
+        tw = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            back = lqcd.LatticeStateEFSyncDefault.read(path, 4, F.FORMAT_BINCODE, lqcd.F32)
+            del back
+        tr = time.perf_counter() - t0
+    cfg = {"workload": ("LatticeStateEFSyncDefault of a 32^4 D=4 f32 lattice written and read back in the "
+                        "serde bincode layout (links and E as f64, 872 MB per file), temp-dir file"),
+           "lattice": [L] * 4, "file_bytes": nbytes}
+    return contract_line(
+        args, "state checkpoint write+read GB/s, 32^4", 2 * nbytes * args.steps / (tw + tr) / 1e9, "GB/s",
+        args.steps, tw + tr, "f64", cfg, None, None,
+        write_GB_s=round(nbytes * args.steps / tw / 1e9, 3), read_GB_s=round(nbytes * args.steps / tr / 1e9, 3))
+
+
 def bench_small(args, lqcd):
     F = lqcd._ffi
     L, beta, spread = 8, 6.0, 0.1
@@ -555,7 +587,8 @@ def main():
         return
     if args.workload != "hmc32":
         run = {"batched": bench_batched, "large": bench_large, "mh3d": bench_mh3d,
-               "small": bench_small, "fields": bench_fields}[args.workload]
+               "small": bench_small, "fields": bench_fields,
+               "state": bench_state}[args.workload]
         print(json.dumps(run(args, lqcd)))
         return
 

@@ -47,7 +47,8 @@ typedef enum {
     LQCD_ERR_DEVICE = 8,
     LQCD_ERR_OUT_OF_MEMORY = 9,
     /* StateInitializationError::GaussProjectionError (src/error.rs:131-132) */
-    LQCD_ERR_GAUSS_PROJECTION = 10
+    LQCD_ERR_GAUSS_PROJECTION = 10,
+    LQCD_ERR_IO = 11
 } lqcd_status;
 
 typedef enum { LQCD_F64 = 0, LQCD_F32 = 1 } lqcd_precision;
@@ -248,6 +249,33 @@ int lqcd_magnetic_field_vec(lqcd_lattice* lat, double* out, size_t n);
  * the result on the device.  No counterpart in the reference. */
 enum { LQCD_FIELD_CLOVER = 0, LQCD_FIELD_F_MU_NU = 1, LQCD_FIELD_MAGNETIC = 2 };
 int lqcd_field_strength_device(lqcd_lattice* lat, int what, double* d_out, size_t n);
+
+/* ------------------------------------------------------ state interchange */
+/* The serde data model of LatticeStateDefault<D> (src/simulation/state.rs:653-659)
+ *   { lattice: { size, dim }, beta, link_matrix: { data: [matrix; n] } }
+ * and of LatticeStateEFSyncDefault<LatticeStateDefault<D>, D> (state.rs:1046-1062)
+ *   { e_field: { data: [[ { data: [8 reals] }; D ]; nsite] }, t, lattice_state },
+ * a matrix being the tuple of its 9 entries in column-major order and an entry
+ * the tuple (re, im) (nalgebra 0.31 / num-complex 0.4 serde), in one of two
+ * serde formats: bincode 1.x `serialize` (little-endian fixed-width integers,
+ * u64 sequence lengths) or serde_json.  D is not serialized: readers take it
+ * from the caller and check the lengths against dim^D.  Only hypercubic
+ * lattices (the reference's LatticeCyclic) can be written. */
+enum { LQCD_FORMAT_BINCODE = 0, LQCD_FORMAT_JSON = 1 };
+int lqcd_state_write(lqcd_lattice* lat, double beta, const char* path, int format);
+int lqcd_state_ef_write(lqcd_lattice* lat, double beta, uint64_t t, const char* path, int format);
+/* parses a whole file (with_efield: the EFSync layout) and returns its lattice
+ * spacing, points per direction, beta and t (t may be NULL); needs no device */
+int lqcd_state_read_header(const char* path, int format, int with_efield, int dim, double* size,
+                           size_t* points, double* beta, uint64_t* t);
+/* the same parse into host buffers: links n_links x 18 (link index site*D+mu,
+ * as lqcd_links_download), efield n_links x 8 when with_efield; no device */
+int lqcd_state_read_host(const char* path, int format, int with_efield, int dim, double* links,
+                         size_t n_links, double* efield);
+/* loads the links (and E) of a file into a lattice of the same shape and
+ * spacing; beta and t (may be NULL) receive the stored values */
+int lqcd_state_read(lqcd_lattice* lat, const char* path, int format, int with_efield, double* beta,
+                    uint64_t* t);
 
 /* LatticeStateDefault::normalize_link_matrices (src/simulation/state.rs:754-756) */
 int lqcd_normalize_link_matrices(lqcd_lattice* lat);

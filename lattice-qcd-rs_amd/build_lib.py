@@ -17,7 +17,7 @@ OUT = os.path.join(HERE, "lqcd", "liblqcd_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LQCD_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["capi.cpp", "capi_batch.cpp", "k_fields.hip", "k_observables.hip", "k_md.hip", "k_mc.hip",
+SOURCES = ["capi.cpp", "capi_batch.cpp", "capi_state.cpp", "k_fields.hip", "k_observables.hip", "k_md.hip", "k_mc.hip",
            "k_batch.hip", "k_gauss.hip", "k_clover.hip"]
 
 FLAGS = [

@@ -183,6 +183,15 @@ class _DeviceLattice:
             pass
 
 
+def _read_header(path, format, with_e, d):
+    size, points, beta, t = C.c_double(), C.c_size_t(), C.c_double(), C.c_uint64()
+    _raise(F.lib().lqcd_state_read_header(str(path).encode(), format, int(with_e), d, C.byref(size),
+                                          C.byref(points), C.byref(beta), C.byref(t)))
+    if with_e:
+        return size.value, points.value, beta.value, t.value
+    return size.value, points.value, beta.value
+
+
 class LatticeStateDefault:
     """``LatticeStateDefault<D>`` (src/simulation/state.rs:653-850), resident in HBM.
 
@@ -213,6 +222,20 @@ class LatticeStateDefault:
                         precision: int = F64, ctx: Optional[Context] = None):
         st = cls(LatticeCyclic(size, number_of_points, d), beta, None, precision, ctx)
         _raise(F.lib().lqcd_links_random(st.handle, C.c_uint64(seed)))
+        return st
+
+    # serde interchange (Serialize / Deserialize derives, state.rs:653-659):
+    # bincode 1.x or serde_json of { lattice, beta, link_matrix }
+    def write(self, path: str, format: int = F.FORMAT_BINCODE) -> None:
+        _raise(F.lib().lqcd_state_write(self.handle, self._beta, str(path).encode(), format))
+
+    @classmethod
+    def read(cls, path: str, d: int = 4, format: int = F.FORMAT_BINCODE, precision: int = F64,
+             ctx: Optional[Context] = None):
+        size, points, beta = _read_header(path, format, False, d)
+        st = cls(LatticeCyclic(size, points, d), beta, None, precision, ctx)
+        b = C.c_double()
+        _raise(F.lib().lqcd_state_read(st.handle, str(path).encode(), format, 0, C.byref(b), None))
         return st
 
     @property
@@ -349,6 +372,21 @@ class LatticeStateEFSyncDefault:
     @classmethod
     def new_e_cold(cls, lattice_state: LatticeStateDefault):
         return cls(lattice_state, None)
+
+    # serde interchange (state.rs:1046-1062): { e_field, t, lattice_state }
+    def write(self, path: str, format: int = F.FORMAT_BINCODE) -> None:
+        _raise(F.lib().lqcd_state_ef_write(self.handle, self._state.beta(), C.c_uint64(self._t),
+                                           str(path).encode(), format))
+
+    @classmethod
+    def read(cls, path: str, d: int = 4, format: int = F.FORMAT_BINCODE, precision: int = F64,
+             ctx: Optional[Context] = None):
+        size, points, beta, t = _read_header(path, format, True, d)
+        st = LatticeStateDefault(LatticeCyclic(size, points, d), beta, None, precision, ctx)
+        ef = cls(st, None, t)
+        b, tt = C.c_double(), C.c_uint64()
+        _raise(F.lib().lqcd_state_read(st.handle, str(path).encode(), format, 1, C.byref(b), C.byref(tt)))
+        return ef
 
     @classmethod
     def new_random_e_state(cls, lattice_state: LatticeStateDefault, seed: int, counter: int = 0,

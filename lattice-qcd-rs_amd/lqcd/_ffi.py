@@ -25,6 +25,8 @@ ERR_UNSUPPORTED = 7
 ERR_DEVICE = 8
 ERR_OUT_OF_MEMORY = 9
 ERR_GAUSS_PROJECTION = 10
+ERR_IO = 11
+FORMAT_BINCODE, FORMAT_JSON = 0, 1
 
 F64, F32 = 0, 1
 DRIFT_ADDITIVE, DRIFT_EXP = 0, 1
@@ -132,6 +134,12 @@ SIGNATURES = {
     "lqcd_f_mu_nu_field": (_I, [_P, _P, _S]),
     "lqcd_magnetic_field_vec": (_I, [_P, _P, _S]),
     "lqcd_field_strength_device": (_I, [_P, _I, _P, _S]),
+    "lqcd_state_write": (_I, [_P, _D, C.c_char_p, _I]),
+    "lqcd_state_ef_write": (_I, [_P, _D, _U64, C.c_char_p, _I]),
+    "lqcd_state_read_header": (_I, [C.c_char_p, _I, _I, _I, C.POINTER(C.c_double), C.POINTER(C.c_size_t),
+                                    C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "lqcd_state_read_host": (_I, [C.c_char_p, _I, _I, _I, _P, _S, _P]),
+    "lqcd_state_read": (_I, [_P, C.c_char_p, _I, _I, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "lqcd_overrelaxation_reverse_sweep": (_I, [_P]),
     "lqcd_normalize_link_matrices": (_I, [_P]),
     "lqcd_stream_copy": (_I, [_P, _S, _I, C.POINTER(_D)]),
