@@ -47,6 +47,9 @@ METRIC = ("link updates/s + % HBM peak; FP32-mixed vs FP64 HMC traj/s at matched
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, AMD specification
 REALS_KICK_DRIFT = 52   # read U (18) + E (8), write E (8) + U (18) per link
 REALS_KICK = 34         # read U (18) + E (8), write E (8)
+# with two-row link reads (f32, exp drift, links in SU(3)) U is read as 12 reals
+REALS_KICK_DRIFT_R12 = 46
+REALS_KICK_R12 = 28
 REALS_PLAQ = 18
 REALS_EENERGY = 8
 REALS_REUNIT = 36
@@ -123,6 +126,7 @@ def run_leg(lqcd, precision, args, steps, warmup, profile):
     nlink = st.lattice().number_of_canonical_links_space()
     return {
         "md_variant": st.md_variant(),
+        "r12": st.md_reconstruct_last(),
         "seconds": t1 - t0,
         "traj_per_s": steps / (t1 - t0),
         "mean_abs_dh": float(sum(abs(x) for x in dh) / len(dh)),
@@ -134,13 +138,15 @@ def run_leg(lqcd, precision, args, steps, warmup, profile):
     }
 
 
-def measured_traffic(variant):
+def measured_traffic(variant, r12):
     """HBM bytes per kick+drift launch (fetched + written) from the committed
-    rocprofv3 counter summary, when it covers the selected MD variant."""
+    rocprofv3 counter summary, when it covers the selected MD variant and
+    link-read mode."""
     path = os.path.join(ROOT, "profiles", "kick_drift_32x4_f32_pmc.md")
+    key = f"traffic_bytes_per_launch variant={variant}{' r12' if r12 else ''}:"
     try:
         for line in open(path):
-            if line.startswith(f"traffic_bytes_per_launch variant={variant}:"):
+            if line.startswith(key):
                 return int(line.split(":")[1])
     except OSError:
         pass
@@ -360,10 +366,12 @@ def hmc_leg(lqcd, precision, extents, beta, dt, md_steps, steps, warmup, seed, c
     seconds = time.perf_counter() - t0
     nlink = st.lattice().number_of_canonical_links_space()
     elem = 4 if precision == lqcd.F32 else 8
-    kd = kernel_rate(ctx, F.KERNEL_KICK_DRIFT, nlink * REALS_KICK_DRIFT * elem)
+    r12 = st.md_reconstruct_last()
+    kd = kernel_rate(ctx, F.KERNEL_KICK_DRIFT,
+                     nlink * (REALS_KICK_DRIFT_R12 if r12 else REALS_KICK_DRIFT) * elem)
     ctx.profile_enable(0)
     return {"seconds": seconds, "nlink": nlink, "plaquette": plaq, "dh": dh, "acceptance": acc / steps,
-            "kick_drift": kd, "md_variant": st.md_variant()}
+            "kick_drift": kd, "md_variant": st.md_variant(), "r12": r12}
 
 
 def bench_large(args, lqcd):
@@ -374,11 +382,12 @@ def bench_large(args, lqcd):
                         "exp link update, FP32 links/E with FP64 dH/plaquette/accept/reunitarisation, "
                         "plaquette after every trajectory"),
            "lattice": ext, "links": leg["nlink"], "md_steps": md, "dt": dt, "beta": beta,
-           "md_variant": leg["md_variant"]}
+           "md_variant": leg["md_variant"], "two_row_link_reads": leg["r12"]}
     cpu = cpu_md_sample([24, 24, 24, 48], beta, dt, args.cpu_seconds, args.seed) if args.cpu_md_steps > 0 else None
+    kname = "MD kick+drift (fused force + kick + exp drift" + (", links read as two rows)" if leg["r12"] else ")")
     return contract_line(
         args, "link updates/s, 48^3x96 FP32-mixed HMC", value, "link updates/s", args.steps, leg["seconds"],
-        "f32", cfg, roofline_from(leg["kick_drift"], "MD kick+drift (fused force + kick + exp drift)"), cpu,
+        "f32", cfg, roofline_from(leg["kick_drift"], kname), cpu,
         traj_per_s=round(args.steps / leg["seconds"], 3), acceptance=leg["acceptance"],
         mean_abs_dH=float(sum(abs(x) for x in leg["dh"]) / len(leg["dh"])),
         plaquette_per_trajectory=[round(p, 10) for p in leg["plaquette"]])
@@ -609,9 +618,11 @@ def main():
         return {"launches": n, "avg_us": round(avg_s * 1e6, 2), "GB_s": round(gbs, 1),
                 "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
-    kd = kernel_line(mixed, F.KERNEL_KICK_DRIFT, REALS_KICK_DRIFT, 4)
+    r12 = mixed["r12"]
+    reals_kd = REALS_KICK_DRIFT_R12 if r12 else REALS_KICK_DRIFT
+    kd = kernel_line(mixed, F.KERNEL_KICK_DRIFT, reals_kd, 4)
     variant = mixed["md_variant"]
-    traffic = measured_traffic(variant) if args.extent == 32 else None
+    traffic = measured_traffic(variant, r12) if args.extent == 32 else None
     roofline = {
         "bound": "hbm",
         "achieved": kd["GB_s"],
@@ -620,8 +631,9 @@ def main():
         "frac": kd["hbm_frac"],
         "traffic": traffic,
         "kernel": (f"k_md_{'site' if variant >= 2 else 'link'}_xcd<4,float,float,EXP> "
-                   f"(fused force + momentum kick + exp link drift, MD variant {variant})"),
-        "bytes_per_launch": nlink * REALS_KICK_DRIFT * 4,
+                   f"(fused force + momentum kick + exp link drift, MD variant {variant}"
+                   f"{', links read as two rows' if r12 else ''})"),
+        "bytes_per_launch": nlink * reals_kd * 4,
     }
     kernels = {}
     for name, kind, reals in [("kick_drift", F.KERNEL_KICK_DRIFT, REALS_KICK_DRIFT),
@@ -630,7 +642,8 @@ def main():
                               ("efield_energy", F.KERNEL_EFIELD_ENERGY, REALS_EENERGY),
                               ("reunitarize", F.KERNEL_REUNIT, REALS_REUNIT),
                               ("momenta", F.KERNEL_MOMENTA, REALS_MOMENTA)]:
-        kernels["f32_" + name] = kernel_line(mixed, kind, reals, 4)
+        reals32 = {"kick_drift": reals_kd, "kick": REALS_KICK_R12 if r12 else REALS_KICK}.get(name, reals)
+        kernels["f32_" + name] = kernel_line(mixed, kind, reals32, 4)
         kernels["f64_" + name] = kernel_line(f64, kind, reals, 8)
 
     cpu = cpu_baseline(args) if args.cpu_md_steps > 0 else None

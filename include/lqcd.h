@@ -294,6 +294,17 @@ int lqcd_normalize_link_matrices(lqcd_lattice* lat);
 int lqcd_md_autotune(lqcd_lattice* lat, int drift, int* variant);
 int lqcd_md_variant_set(lqcd_lattice* lat, int variant);
 int lqcd_md_variant_get(const lqcd_lattice* lat, int* variant);
+/* Two-row link reads in f32 exp-drift integrations.  No counterpart in the
+ * reference.  With mode 1 (default) an integration of an f32 lattice with
+ * LQCD_DRIFT_EXP first measures the distance of the links from SU(3)
+ * (lqcd_su3_distance); when it is at most 64 f32 ulps of 1 the MD kernels
+ * read only the first two rows of every link and rebuild the third as
+ * conj(row0 x row1), exact on SU(3).  Mode 0 reads all nine entries always.
+ * *last_used tells whether the last integration read two rows. */
+int lqcd_md_reconstruct_set(lqcd_lattice* lat, int mode);
+int lqcd_md_reconstruct_get(const lqcd_lattice* lat, int* mode, int* last_used);
+/* max over links of max(max_ij |(U U^+ - 1)_ij|, |det U - 1|), double arithmetic */
+int lqcd_su3_distance(lqcd_lattice* lat, double* out);
 
 /* ------------------------------------------------------ batched chains */
 /* Many independent chains, each a LatticeStateDefault of its own extents,

@@ -111,10 +111,32 @@ int ensure_md_variant(lqcd_lattice* L, int drift)
     return autotune_md(L, drift, nullptr);
 }
 
+// Links whose distance from SU(3) (launch_su3_distance) is at most this are
+// read as two rows by f32 exp-drift integrations: 64 f32 ulps of 1, i.e. the
+// rounding of a stored SU(3) matrix with room for the growth of a trajectory.
+constexpr double kR12Distance = 64.0 * 5.9604644775390625e-8;
+
+struct R12Scope {
+    lqcd_lattice* L;
+    ~R12Scope() { L->md_r12 = false; }
+};
+
 // n steps of the chosen scheme starting from U[cur]; *last gets the buffer
-// holding the final links.  U[cur] is left untouched.
+// holding the final links.  U[cur] is left untouched.  An f32 integration
+// with the exp drift keeps the links in SU(3) (up to rounding), so when the
+// starting links are in SU(3) its kernels read only their first two rows.
 int integrate_steps(lqcd_lattice* L, double dt, size_t n, int drift, int scheme, int* last)
 {
+    R12Scope scope{L};
+    L->md_r12 = false;
+    if (L->precision == LQCD_F32 && drift == LQCD_DRIFT_EXP && L->md_r12_mode) {
+        HIP_TRY(launch_su3_distance(L, L->U[L->cur], L->result + 8));
+        HIP_TRY(hipMemcpyAsync(L->host_result + 8, L->result + 8, sizeof(double), hipMemcpyDeviceToHost,
+                               L->ctx->stream));
+        HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+        L->md_r12 = L->host_result[8] <= kR12Distance;
+    }
+    L->md_r12_last = L->md_r12;
     if (int rc = ensure_md_variant(L, drift)) return rc;
     int w1, w2;
     other_buffers(L->cur, &w1, &w2);
@@ -839,6 +861,32 @@ int lqcd_md_variant_get(const lqcd_lattice* L, int* variant)
 {
     CHECK_ARG(L && variant, "null argument");
     *variant = L->md_variant;
+    return LQCD_OK;
+}
+
+int lqcd_md_reconstruct_set(lqcd_lattice* L, int mode)
+{
+    CHECK_ARG(L, "lattice is null");
+    CHECK_ARG(mode == 0 || mode == 1, "mode must be 0 or 1");
+    L->md_r12_mode = mode;
+    return LQCD_OK;
+}
+
+int lqcd_md_reconstruct_get(const lqcd_lattice* L, int* mode, int* last_used)
+{
+    CHECK_ARG(L && mode && last_used, "null argument");
+    *mode = L->md_r12_mode;
+    *last_used = L->md_r12_last ? 1 : 0;
+    return LQCD_OK;
+}
+
+int lqcd_su3_distance(lqcd_lattice* L, double* out)
+{
+    CHECK_ARG(L && out, "null argument");
+    DeviceGuard dg(L->ctx->device);
+    HIP_TRY(launch_su3_distance(L, L->U[L->cur], L->result));
+    if (int rc = read_results(L, 1)) return rc;
+    *out = L->host_result[0];
     return LQCD_OK;
 }
 

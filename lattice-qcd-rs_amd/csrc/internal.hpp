@@ -42,6 +42,9 @@ struct lqcd_lattice {
     uint32_t* mh_acc = nullptr;    // per-link Metropolis accept flags
     int nblocks_reduce = 0;
     int md_variant = -1;           // work decomposition of the MD kernels (-1: not tuned)
+    int md_r12_mode = 1;           // 1: f32 exp-drift integrations read SU(3) links as two rows
+    bool md_r12 = false;           // set for the duration of one integration that does
+    bool md_r12_last = false;      // whether the last integration did
     void* gauss_g = nullptr;       // Gauss law G(x): 9 planes of double2
     double* gauss_e[2] = {nullptr, nullptr};  // double copies of E for the projection
 };
@@ -136,6 +139,8 @@ hipError_t launch_efield_to_double(const lqcd_lattice* L, const void* E, double*
 hipError_t launch_efield_from_double(const lqcd_lattice* L, const double* Ed, void* E);
 // sum of nblocks partials (slot 0) into d_out1
 hipError_t launch_final_reduce1(const lqcd_lattice* L, int nblocks, double* d_out1);
+// max over links of max(|U U^+ - 1|_ij, |det U - 1|) of buffer U into d_out1
+hipError_t launch_su3_distance(const lqcd_lattice* L, const void* U, double* d_out1);
 
 // profiling hooks (capi.cpp)
 void prof_begin(lqcd_ctx* ctx, int kind, lqcd_prof_event* ev);

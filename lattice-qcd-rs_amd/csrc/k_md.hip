@@ -47,7 +47,7 @@ __global__ __launch_bounds__(kBlock) void k_force_store(const vec2_t<S>* __restr
 }
 
 // VAR_LINK: grid (site blocks, D)
-template <int D, typename S, typename T, int DR>
+template <int D, typename S, typename T, int DR, bool R12>
 __global__ __launch_bounds__(kBlock) void k_md_link(const vec2_t<S>* __restrict__ U_in,
                                                     vec2_t<S>* __restrict__ U_out,
                                                     S* __restrict__ E, Geom g, T fscale, T kick,
@@ -56,11 +56,11 @@ __global__ __launch_bounds__(kBlock) void k_md_link(const vec2_t<S>* __restrict_
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int mu = blockIdx.y;
     if (s >= g.nsite) return;
-    md_link<D, S, T, DR>(U_in, U_out, E, g, mu, s, fscale, kick, dfac);
+    md_link<D, S, T, DR, R12>(U_in, U_out, E, g, mu, s, fscale, kick, dfac);
 }
 
 // VAR_LINK_XCD: 1-D grid of (site block, mu) pairs, mu fastest, XCD remapped
-template <int D, typename S, typename T, int DR>
+template <int D, typename S, typename T, int DR, bool R12>
 __global__ __launch_bounds__(kBlock) void k_md_link_xcd(const vec2_t<S>* __restrict__ U_in,
                                                         vec2_t<S>* __restrict__ U_out,
                                                         S* __restrict__ E, Geom g, T fscale, T kick,
@@ -70,11 +70,11 @@ __global__ __launch_bounds__(kBlock) void k_md_link_xcd(const vec2_t<S>* __restr
     const int mu = (int)(logical % D);
     const int64_t s = (logical / D) * kBlock + threadIdx.x;
     if (s >= g.nsite) return;
-    md_link<D, S, T, DR>(U_in, U_out, E, g, mu, s, fscale, kick, dfac);
+    md_link<D, S, T, DR, R12>(U_in, U_out, E, g, mu, s, fscale, kick, dfac);
 }
 
 // VAR_SITE_XCD / VAR_SITE_SLAB (slab_p > 0): one thread per site, all D links
-template <int D, typename S, typename T, int DR>
+template <int D, typename S, typename T, int DR, bool R12>
 __global__ __launch_bounds__(kBlock) void k_md_site_xcd(const vec2_t<S>* __restrict__ U_in,
                                                         vec2_t<S>* __restrict__ U_out,
                                                         S* __restrict__ E, Geom g, T fscale, T kick,
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(kBlock) void k_md_site_xcd(const vec2_t<S>* __restr
     if (slab_p > 0) logical = slab_block(logical, slab_p, slab_g);
     const int64_t s = logical * kBlock + threadIdx.x;
     if (s >= g.nsite) return;
-    md_site<D, S, T, DR>(U_in, U_out, E, g, s, fscale, kick, dfac);
+    md_site<D, S, T, DR, R12>(U_in, U_out, E, g, s, fscale, kick, dfac);
 }
 
 template <int D, typename S, typename T, int MODE>
@@ -108,8 +108,8 @@ static dim3 link_grid(const lqcd_lattice* L)
     return dim3(grid_for(L->g.nsite), (unsigned)L->g.dim);
 }
 
-template <int D, typename S, int DR>
-static void launch_md_variant(const lqcd_lattice* L, int variant, const void* U_in, void* U_out, void* E,
+template <int D, typename S, int DR, bool R12>
+static void launch_md_variant_r(const lqcd_lattice* L, int variant, const void* U_in, void* U_out, void* E,
                               double kick, double dfac)
 {
     const S fs = (S)md_force_scale(L->size), k = (S)kick, df = (S)dfac;
@@ -128,16 +128,31 @@ static void launch_md_variant(const lqcd_lattice* L, int variant, const void* U_
             slab_p = slice / kBlock;
             slab_g = Ls / 8;
         }
-        k_md_site_xcd<D, S, S, DR><<<nsb, kBlock, 0, st>>>(ui, uo, e, L->g, fs, k, df, slab_p, slab_g);
+        k_md_site_xcd<D, S, S, DR, R12><<<nsb, kBlock, 0, st>>>(ui, uo, e, L->g, fs, k, df, slab_p, slab_g);
         break;
     }
     case VAR_LINK_XCD:
-        k_md_link_xcd<D, S, S, DR><<<nsb * (unsigned)D, kBlock, 0, st>>>(ui, uo, e, L->g, fs, k, df);
+        k_md_link_xcd<D, S, S, DR, R12><<<nsb * (unsigned)D, kBlock, 0, st>>>(ui, uo, e, L->g, fs, k, df);
         break;
     default:
-        k_md_link<D, S, S, DR><<<link_grid(L), kBlock, 0, st>>>(ui, uo, e, L->g, fs, k, df);
+        k_md_link<D, S, S, DR, R12><<<link_grid(L), kBlock, 0, st>>>(ui, uo, e, L->g, fs, k, df);
         break;
     }
+}
+
+// links read as two rows (R12) only for f32 storage, when the integration
+// has checked them to be in SU(3) (lqcd_lattice::md_r12)
+template <int D, typename S, int DR>
+static void launch_md_variant(const lqcd_lattice* L, int variant, const void* U_in, void* U_out, void* E,
+                              double kick, double dfac)
+{
+    if constexpr (sizeof(S) == 4 && DR != DRIFT_ADDITIVE) {
+        if (L->md_r12) {
+            launch_md_variant_r<D, S, DR, true>(L, variant, U_in, U_out, E, kick, dfac);
+            return;
+        }
+    }
+    launch_md_variant_r<D, S, DR, false>(L, variant, U_in, U_out, E, kick, dfac);
 }
 
 static int chosen_variant(const lqcd_lattice* L)

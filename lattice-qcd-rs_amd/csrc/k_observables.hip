@@ -61,6 +61,61 @@ __global__ __launch_bounds__(kBlock) void k_final_reduce(const double* __restric
     block_reduce_to<K>(acc, out);
 }
 
+// max over lanes of a wave, then over the waves of the block (thread 0 gets it)
+__device__ __forceinline__ double block_max(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    __shared__ double sh[kBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) sh[wave] = v;
+    __syncthreads();
+    double m = sh[0];
+#pragma unroll
+    for (int w = 1; w < kBlock / 64; ++w) m = fmax(m, sh[w]);
+    return m;
+}
+
+// Distance of the stored links from SU(3): the largest of max_ij |(U U^+ - 1)_ij|
+// and |det U - 1| over all links, in double arithmetic (a NaN entry gives
+// +inf).  Per-block maxima first, then one block over them.
+template <int D, typename S>
+__global__ __launch_bounds__(kBlock) void k_su3_distance_partials(const vec2_t<S>* __restrict__ U, Geom g,
+                                                                  double* __restrict__ partials)
+{
+    double dev = 0.0;
+    const int64_t step = (int64_t)gridDim.x * kBlock;
+    for (int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x; s < g.nsite; s += step)
+#pragma unroll
+        for (int mu = 0; mu < D; ++mu) {
+            const mat3<double> u = load_link<double, S>(U, g.npad, mu, s);
+            mat3<double> p;
+            mul<false, true>(u, u, p);
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const double re = p.e[i * 3 + j].re - (i == j ? 1.0 : 0.0), im = p.e[i * 3 + j].im;
+                    const double d = sqrt(re * re + im * im);
+                    dev = d == d ? fmax(dev, d) : INFINITY;
+                }
+            const cplx<double> dt = det(u);
+            const double d = sqrt((dt.re - 1.0) * (dt.re - 1.0) + dt.im * dt.im);
+            dev = d == d ? fmax(dev, d) : INFINITY;
+        }
+    const double m = block_max(dev);
+    if (threadIdx.x == 0) partials[(int64_t)blockIdx.x * kReduceSlots] = m;
+}
+
+__global__ __launch_bounds__(kBlock) void k_final_max(const double* __restrict__ partials, int nblocks,
+                                                       double* __restrict__ out)
+{
+    double m = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += kBlock) m = fmax(m, partials[(int64_t)b * kReduceSlots]);
+    m = block_max(m);
+    if (threadIdx.x == 0) out[0] = m;
+}
+
 static int reduce_blocks(int64_t n)
 {
     int64_t b = (n + kBlock - 1) / kBlock;
@@ -78,6 +133,15 @@ hipError_t launch_plaquette_sum(const lqcd_lattice* L, const void* U, double* d_
                             (const vec2_t<S>*)U, L->g, L->partials));
     prof_end(L->ctx, &ev);
     k_final_reduce<3><<<1, kBlock, 0, L->ctx->stream>>>(L->partials, nb, d_out3);
+    return hipGetLastError();
+}
+
+hipError_t launch_su3_distance(const lqcd_lattice* L, const void* U, double* d_out1)
+{
+    const int nb = reduce_blocks(L->g.nsite);
+    LQCD_DISPATCH_DS(L, k_su3_distance_partials<D, S><<<nb, kBlock, 0, L->ctx->stream>>>(
+                            (const vec2_t<S>*)U, L->g, L->partials));
+    k_final_max<<<1, kBlock, 0, L->ctx->stream>>>(L->partials, nb, d_out1);
     return hipGetLastError();
 }
 

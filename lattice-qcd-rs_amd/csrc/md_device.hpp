@@ -33,7 +33,7 @@ __device__ __forceinline__ int64_t slab_block(int64_t j, int64_t P, int64_t G)
     return (q * G + r % G) * P + r / G;
 }
 
-template <int D, typename S, typename T>
+template <int D, typename S, typename T, bool R12 = false>
 __device__ __forceinline__ void staple_sum(const vec2_t<S>* __restrict__ U, const Geom& g, int mu,
                                            int64_t s, const Neighbours<D>& nb, mat3<T>& sigma)
 {
@@ -44,21 +44,21 @@ __device__ __forceinline__ void staple_sum(const vec2_t<S>* __restrict__ U, cons
         if (nu == mu) continue;
         // upper staple: U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
         {
-            const mat3<T> a = load_link<T, S>(U, g.npad, nu, s_pmu);
-            const mat3<T> b = load_link<T, S>(U, g.npad, mu, s + nb.up[nu]);
+            const mat3<T> a = load_link_sel<R12, T, S>(U, g.npad, nu, s_pmu);
+            const mat3<T> b = load_link_sel<R12, T, S>(U, g.npad, mu, s + nb.up[nu]);
             mat3<T> ab;
             mul<false, true>(a, b, ab);
-            const mat3<T> c = load_link<T, S>(U, g.npad, nu, s);
+            const mat3<T> c = load_link_sel<R12, T, S>(U, g.npad, nu, s);
             mul_add<false, true>(ab, c, sigma);
         }
         // lower staple: U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)
         {
             const int64_t s_mnu = s + nb.dn[nu];
-            const mat3<T> a = load_link<T, S>(U, g.npad, nu, s_pmu + nb.dn[nu]);
-            const mat3<T> b = load_link<T, S>(U, g.npad, mu, s_mnu);
+            const mat3<T> a = load_link_sel<R12, T, S>(U, g.npad, nu, s_pmu + nb.dn[nu]);
+            const mat3<T> b = load_link_sel<R12, T, S>(U, g.npad, mu, s_mnu);
             mat3<T> ab;
             mul<true, true>(a, b, ab);
-            const mat3<T> c = load_link<T, S>(U, g.npad, nu, s_mnu);
+            const mat3<T> c = load_link_sel<R12, T, S>(U, g.npad, nu, s_mnu);
             mul_add<false, false>(ab, c, sigma);
         }
     }
@@ -67,7 +67,7 @@ __device__ __forceinline__ void staple_sum(const vec2_t<S>* __restrict__ U, cons
 // All D staple sums of site s.  Upper staples pairwise through
 // W = U_nu(x+mu) U_mu(x+nu)^+:  Sigma_mu += W U_nu(x)^+,
 // Sigma_nu += W^+ U_mu(x)^+.
-template <int D, typename S, typename T>
+template <int D, typename S, typename T, bool R12 = false>
 __device__ __forceinline__ void staple_sums_site(const vec2_t<S>* __restrict__ U, const Geom& g,
                                                  int64_t s, const Neighbours<D>& nb,
                                                  mat3<T> (&sig)[D])
@@ -80,28 +80,28 @@ __device__ __forceinline__ void staple_sums_site(const vec2_t<S>* __restrict__ U
         for (int nu = mu + 1; nu < D; ++nu) {
             mat3<T> w;
             {
-                const mat3<T> a = load_link<T, S>(U, g.npad, nu, s + nb.up[mu]);
-                const mat3<T> b = load_link<T, S>(U, g.npad, mu, s + nb.up[nu]);
+                const mat3<T> a = load_link_sel<R12, T, S>(U, g.npad, nu, s + nb.up[mu]);
+                const mat3<T> b = load_link_sel<R12, T, S>(U, g.npad, mu, s + nb.up[nu]);
                 mul<false, true>(a, b, w);
             }
             {
-                const mat3<T> c = load_link<T, S>(U, g.npad, nu, s);
+                const mat3<T> c = load_link_sel<R12, T, S>(U, g.npad, nu, s);
                 mul_add<false, true>(w, c, sig[mu]);
             }
             {
-                const mat3<T> c = load_link<T, S>(U, g.npad, mu, s);
+                const mat3<T> c = load_link_sel<R12, T, S>(U, g.npad, mu, s);
                 mul_add<true, true>(w, c, sig[nu]);
             }
         }
 #pragma unroll
     for (int nu = 0; nu < D; ++nu) {
         const int64_t s_mnu = s + nb.dn[nu];
-        const mat3<T> c = load_link<T, S>(U, g.npad, nu, s_mnu);
+        const mat3<T> c = load_link_sel<R12, T, S>(U, g.npad, nu, s_mnu);
 #pragma unroll
         for (int mu = 0; mu < D; ++mu) {
             if (mu == nu) continue;
-            const mat3<T> a = load_link<T, S>(U, g.npad, nu, s_mnu + nb.up[mu]);
-            const mat3<T> b = load_link<T, S>(U, g.npad, mu, s_mnu);
+            const mat3<T> a = load_link_sel<R12, T, S>(U, g.npad, nu, s_mnu + nb.up[mu]);
+            const mat3<T> b = load_link_sel<R12, T, S>(U, g.npad, mu, s_mnu);
             mat3<T> ab;
             mul<true, true>(a, b, ab);
             mul_add<false, false>(ab, c, sig[mu]);
@@ -155,21 +155,22 @@ __device__ __forceinline__ void finish_link(const mat3<T>& u, const mat3<T>& sig
     }
 }
 
-// one link (mu, s): e += kick * F(U_in); U_out = drift(U_in, e) unless DR < 0
-template <int D, typename S, typename T, int DR>
+// one link (mu, s): e += kick * F(U_in); U_out = drift(U_in, e) unless DR < 0.
+// R12: links read as two rows (load_link_r12), for links in SU(3).
+template <int D, typename S, typename T, int DR, bool R12 = false>
 __device__ __forceinline__ void md_link(const vec2_t<S>* __restrict__ U_in, vec2_t<S>* __restrict__ U_out,
                                         S* __restrict__ E, const Geom& g, int mu, int64_t s, T fscale,
                                         T kick, T dfac)
 {
     const Neighbours<D> nb = neighbours<D>(g, s);
     mat3<T> sigma;
-    staple_sum<D, S, T>(U_in, g, mu, s, nb, sigma);
-    const mat3<T> u = load_link<T, S>(U_in, g.npad, mu, s);
+    staple_sum<D, S, T, R12>(U_in, g, mu, s, nb, sigma);
+    const mat3<T> u = load_link_sel<R12, T, S>(U_in, g.npad, mu, s);
     finish_link<DR, S, T>(u, sigma, U_out, E, g, mu, s, fscale, kick, dfac);
 }
 
 // all D links of site s, sharing the staple loads
-template <int D, typename S, typename T, int DR>
+template <int D, typename S, typename T, int DR, bool R12 = false>
 __device__ __forceinline__ void md_site(const vec2_t<S>* __restrict__ U_in, vec2_t<S>* __restrict__ U_out,
                                         S* __restrict__ E, const Geom& g, int64_t s, T fscale, T kick,
                                         T dfac)
@@ -179,10 +180,10 @@ __device__ __forceinline__ void md_site(const vec2_t<S>* __restrict__ U_in, vec2
     T f[D][8];
     {
         mat3<T> sig[D];
-        staple_sums_site<D, S, T>(U_in, g, s, nb, sig);
+        staple_sums_site<D, S, T, R12>(U_in, g, s, nb, sig);
 #pragma unroll
         for (int mu = 0; mu < D; ++mu) {
-            const mat3<T> u = load_link<T, S>(U_in, g.npad, mu, s);
+            const mat3<T> u = load_link_sel<R12, T, S>(U_in, g.npad, mu, s);
             mat3<T> m;
             mul<false, false>(u, sig[mu], m);
             im_trace_generators(m, f[mu]);
@@ -201,7 +202,7 @@ __device__ __forceinline__ void md_site(const vec2_t<S>* __restrict__ U_in, vec2
     if (DR >= 0) {
 #pragma unroll
         for (int mu = 0; mu < D; ++mu) {
-            const mat3<T> u = load_link<T, S>(U_in, g.npad, mu, s);
+            const mat3<T> u = load_link_sel<R12, T, S>(U_in, g.npad, mu, s);
             mat3<T> un;
             drift_link<T, DR>(u, e[mu], dfac, un);
             store_link<T, S, kCacheStream>(U_out, g.npad, mu, s, un);

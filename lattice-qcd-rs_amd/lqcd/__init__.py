@@ -224,6 +224,22 @@ class LatticeStateDefault:
         _raise(F.lib().lqcd_links_random(st.handle, C.c_uint64(seed)))
         return st
 
+    def set_md_reconstruct(self, mode: int) -> None:
+        """1 (default): f32 exp-drift integrations read SU(3) links as two rows;
+        0: always nine entries."""
+        _raise(F.lib().lqcd_md_reconstruct_set(self.handle, mode))
+
+    def md_reconstruct_last(self) -> bool:
+        mode, used = C.c_int(), C.c_int()
+        _raise(F.lib().lqcd_md_reconstruct_get(self.handle, C.byref(mode), C.byref(used)))
+        return bool(used.value)
+
+    def su3_distance(self) -> float:
+        """max over links of max(|U U^+ - 1|_ij, |det U - 1|)"""
+        out = C.c_double()
+        _raise(F.lib().lqcd_su3_distance(self.handle, C.byref(out)))
+        return float(out.value)
+
     # serde interchange (Serialize / Deserialize derives, state.rs:653-659):
     # bincode 1.x or serde_json of { lattice, beta, link_matrix }
     def write(self, path: str, format: int = F.FORMAT_BINCODE) -> None:

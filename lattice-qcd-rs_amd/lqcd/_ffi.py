@@ -134,6 +134,9 @@ SIGNATURES = {
     "lqcd_f_mu_nu_field": (_I, [_P, _P, _S]),
     "lqcd_magnetic_field_vec": (_I, [_P, _P, _S]),
     "lqcd_field_strength_device": (_I, [_P, _I, _P, _S]),
+    "lqcd_md_reconstruct_set": (_I, [_P, _I]),
+    "lqcd_md_reconstruct_get": (_I, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "lqcd_su3_distance": (_I, [_P, C.POINTER(C.c_double)]),
     "lqcd_state_write": (_I, [_P, _D, C.c_char_p, _I]),
     "lqcd_state_ef_write": (_I, [_P, _D, _U64, C.c_char_p, _I]),
     "lqcd_state_read_header": (_I, [C.c_char_p, _I, _I, _I, C.POINTER(C.c_double), C.POINTER(C.c_size_t),

@@ -44,6 +44,7 @@ def single_run(spec, d, precision, variant, dt, n_steps, n_traj, drift, scheme, 
     st = lqcd.LatticeStateDefault.new_determinist(spec.size, spec.beta, spec.extents, spec.seed, d,
                                                   precision)
     st.set_md_variant(variant)
+    st.set_md_reconstruct(0)   # the batch kernels read all nine entries
     ef = lqcd.LatticeStateEFSyncDefault(st)
     hmc = lqcd.HybridMonteCarloDiagnostic(dt, n_steps, lqcd.SymplecticEulerGpu(drift, scheme), 1)
     stats = []
