@@ -297,23 +297,24 @@ def bench_batched(args, lqcd):
     nlink = sum(b.number_of_links(i) for i in range(len(specs)))
     # eager pass with a timing event around every launch
     b.set_graph(False)
-    b.hmc(dt, n_steps, 1)
+    b.hmc(dt, n_steps, 1, reunitarize=True)
     ctx.profile_enable((1 << F.KERNEL_KICK_DRIFT) | (1 << F.KERNEL_KICK) | (1 << F.KERNEL_PLAQUETTE)
                        | (1 << F.KERNEL_MOMENTA))
     t0 = time.perf_counter()
-    b.hmc(dt, n_steps, 2)
+    b.hmc(dt, n_steps, 2, reunitarize=True)
     eager_s = (time.perf_counter() - t0) / 2
-    kd = kernel_rate(ctx, F.KERNEL_KICK_DRIFT, nlink * REALS_KICK_DRIFT * 4)
-    kick = kernel_rate(ctx, F.KERNEL_KICK, nlink * REALS_KICK * 4)
+    r12 = b.md_reconstruct_last()
+    kd = kernel_rate(ctx, F.KERNEL_KICK_DRIFT, nlink * (REALS_KICK_DRIFT_R12 if r12 else REALS_KICK_DRIFT) * 4)
+    kick = kernel_rate(ctx, F.KERNEL_KICK, nlink * (REALS_KICK_R12 if r12 else REALS_KICK) * 4)
     sums = kernel_rate(ctx, F.KERNEL_PLAQUETTE, nlink * (REALS_PLAQ + REALS_EENERGY) * 4)
     mom = kernel_rate(ctx, F.KERNEL_MOMENTA, nlink * REALS_MOMENTA * 4)
     ctx.profile_enable(0)
     # graph replays
     b.set_graph(True)
-    b.hmc(dt, n_steps, max(args.warmup, 1))
+    b.hmc(dt, n_steps, max(args.warmup, 1), reunitarize=True)
     ctx.synchronize()
     t0 = time.perf_counter()
-    stats = b.hmc(dt, n_steps, args.steps)
+    stats = b.hmc(dt, n_steps, args.steps, reunitarize=True)
     seconds = time.perf_counter() - t0
     value = nlink * n_steps * args.steps / seconds
     groups = {}
@@ -326,13 +327,15 @@ def bench_batched(args, lqcd):
         + (" persistent" if config >= 6 else " full-grid")
     cfg = {"workload": ("64 chains (16 each of 8^4, 12^4, 16^4, 20^4), beta 5.6..6.2, FP32 links/E with "
                         "FP64 dH, hot start per chain seed, 20 leapfrog steps dt=0.01 per trajectory, "
-                        "exp link update, trajectories replayed from a captured HIP graph"),
+                        "exp link update, reunitarisation after every trajectory, trajectories replayed "
+                        "from a captured HIP graph"),
            "chains": 64, "sites": b.total_sites(), "links": nlink, "md_steps": n_steps, "dt": dt,
-           "launch_shape": {"config": config, "shape": shape}}
+           "launch_shape": {"config": config, "shape": shape}, "two_row_link_reads": r12}
     cpu = cpu_md_sample([16] * 4, 5.9, dt, args.cpu_seconds, args.seed) if args.cpu_md_steps > 0 else None
     return contract_line(
         args, "link updates/s, 64 batched chains", value, "link updates/s", args.steps, seconds, "f32",
-        cfg, roofline_from(kd, "k_batch_md<4,float,EXP> (fused force + kick + exp drift, all chains)"), cpu,
+        cfg, roofline_from(kd, "k_batch_md<4,float,EXP> (fused force + kick + exp drift, all chains"
+                               + (", links read as two rows)" if r12 else ")")), cpu,
         chain_traj_per_s=round(64 * args.steps / seconds, 2),
         graph_traj_ms=round(1e3 * seconds / args.steps, 3), eager_traj_ms=round(1e3 * eager_s, 3),
         kernels={"kick_drift": kd, "kick": kick, "sums": sums, "momenta": mom}, by_size=groups)

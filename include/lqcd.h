@@ -375,6 +375,12 @@ int lqcd_batch_config_set(lqcd_batch* batch, int config);
 int lqcd_batch_config_get(const lqcd_batch* batch, int* config);
 /* 1 (default): replay a captured graph per trajectory; 0: launch eagerly */
 int lqcd_batch_graph_enable(lqcd_batch* batch, int enable);
+/* as lqcd_md_reconstruct_set / _get for the chains: lqcd_batch_hmc with f32,
+ * LQCD_DRIFT_EXP and reunitarize set reads the links as two rows when all
+ * chains' links are within 64 f32 ulps of SU(3).  No counterpart in the
+ * reference. */
+int lqcd_batch_reconstruct_set(lqcd_batch* batch, int mode);
+int lqcd_batch_reconstruct_get(const lqcd_batch* batch, int* mode, int* last_used);
 
 /* ------------------------------------------------------------ profiling */
 /* Kernel kinds for lqcd_profile_read (HIP events around each launch). */

@@ -79,7 +79,9 @@ struct lqcd_batch {
     hipGraphExec_t graph = nullptr;        // captured trajectory
     double g_dt = 0.0;                     // what the captured trajectory was built for
     size_t g_steps = 0;
-    int g_drift = -1, g_scheme = -1, g_reunit = -1, g_config = -1;
+    int g_drift = -1, g_scheme = -1, g_reunit = -1, g_config = -1, g_r12 = -1;
+    int md_r12_mode = 1;                   // as lqcd_lattice::md_r12_mode
+    bool md_r12 = false;                   // the current lqcd_batch_hmc reads links as two rows
 };
 
 namespace lqcd {
@@ -90,6 +92,8 @@ hipError_t launch_batch_md(const lqcd_batch* b, int config, const void* U_in, vo
 // settles the grid of launch_batch_md(config, op) without launching
 hipError_t prepare_batch_md(const lqcd_batch* b, int config, int op);
 hipError_t launch_batch_momenta(const lqcd_batch* b);
+// largest distance from SU(3) over all chains' links in U, into d_out1
+hipError_t launch_batch_su3_distance(const lqcd_batch* b, const void* U, double* d_out1);
 hipError_t launch_batch_hot_start(const lqcd_batch* b);
 hipError_t launch_batch_cold(const lqcd_batch* b);
 // per chain: [Re sum Tr P, Im sum Tr P, sum (1 - Re Tr P / 3), sum e^2 / 2] (E may be null)

@@ -111,11 +111,6 @@ int ensure_md_variant(lqcd_lattice* L, int drift)
     return autotune_md(L, drift, nullptr);
 }
 
-// Links whose distance from SU(3) (launch_su3_distance) is at most this are
-// read as two rows by f32 exp-drift integrations: 64 f32 ulps of 1, i.e. the
-// rounding of a stored SU(3) matrix with room for the growth of a trajectory.
-constexpr double kR12Distance = 64.0 * 5.9604644775390625e-8;
-
 struct R12Scope {
     lqcd_lattice* L;
     ~R12Scope() { L->md_r12 = false; }

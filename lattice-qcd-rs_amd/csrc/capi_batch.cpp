@@ -134,6 +134,7 @@ int capture_trajectory(lqcd_batch* b, double dt, size_t n, int drift, int scheme
     b->g_dt = dt;
     b->g_steps = n;
     b->g_drift = drift;
+    b->g_r12 = b->md_r12 ? 1 : 0;
     b->g_scheme = scheme;
     b->g_reunit = reunit;
     b->g_config = b->config;
@@ -426,6 +427,15 @@ int lqcd_batch_hmc(lqcd_batch* b, double dt, size_t n_steps, int drift, int sche
     CHECK_ARG(scheme == LQCD_SCHEME_REFERENCE || scheme == LQCD_SCHEME_LEAPFROG, "unknown scheme");
     if (n_traj == 0) return LQCD_OK;
     DeviceGuard dg(b->ctx->device);
+    // two-row link reads: f32, exp drift, reunitarised trajectories (every
+    // trajectory then starts from links in SU(3)) and links in SU(3) now
+    b->md_r12 = false;
+    if (b->precision == LQCD_F32 && drift == LQCD_DRIFT_EXP && reunitarize && b->md_r12_mode) {
+        HIP_TRY(launch_batch_su3_distance(b, b->U[0], b->d_scratch));
+        HIP_TRY(hipMemcpyAsync(b->h_scratch, b->d_scratch, sizeof(double), hipMemcpyDeviceToHost, b->ctx->stream));
+        HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+        b->md_r12 = b->h_scratch[0] <= kR12Distance;
+    }
     if (int rc = ensure_config(b, drift)) return rc;
     const size_t nc = b->chains.size();
     if (stats)
@@ -433,7 +443,8 @@ int lqcd_batch_hmc(lqcd_batch* b, double dt, size_t n_steps, int drift, int sche
     const int reunit = reunitarize ? 1 : 0;
     if (b->use_graph) {
         const bool stale = !b->graph || b->g_dt != dt || b->g_steps != n_steps || b->g_drift != drift ||
-                           b->g_scheme != scheme || b->g_reunit != reunit || b->g_config != b->config;
+                           b->g_scheme != scheme || b->g_reunit != reunit || b->g_config != b->config ||
+                           b->g_r12 != (b->md_r12 ? 1 : 0);
         if (stale)
             if (int rc = capture_trajectory(b, dt, n_steps, drift, scheme, reunit)) return rc;
     }
@@ -471,6 +482,22 @@ int lqcd_batch_config_get(const lqcd_batch* b, int* config)
 {
     CHECK_ARG(b && config, "null argument");
     *config = b->config;
+    return LQCD_OK;
+}
+
+int lqcd_batch_reconstruct_set(lqcd_batch* b, int mode)
+{
+    CHECK_ARG(b, "batch is null");
+    CHECK_ARG(mode == 0 || mode == 1, "mode must be 0 or 1");
+    b->md_r12_mode = mode;
+    return LQCD_OK;
+}
+
+int lqcd_batch_reconstruct_get(const lqcd_batch* b, int* mode, int* last_used)
+{
+    CHECK_ARG(b && mode && last_used, "null argument");
+    *mode = b->md_r12_mode;
+    *last_used = b->md_r12 ? 1 : 0;
     return LQCD_OK;
 }
 

@@ -57,6 +57,10 @@ constexpr int kMaxReduceBlocks = 4096;
 // device Metropolis proposals use Philox blocks 64 + 8 j + (0..6), below the
 // heat-bath blocks (32768..)
 constexpr size_t kMaxMhUpdates = 4000;
+// Links whose distance from SU(3) (launch_su3_distance) is at most this are
+// read as two rows by f32 exp-drift integrations: 64 f32 ulps of 1, i.e. the
+// rounding of a stored SU(3) matrix with room for the growth of a trajectory.
+constexpr double kR12Distance = 64.0 * 5.9604644775390625e-8;
 
 enum ProfileKind {
     PROF_KICK_DRIFT = 0,

@@ -25,7 +25,7 @@ constexpr uint32_t kAcceptBlock = 62;
 // MD over all chains.  Work units are tiles (SITE) or (tile, mu) pairs, mu
 // fastest; workgroup wg (XCD-remapped) owns the contiguous unit range
 // [wg * chunk, (wg + 1) * chunk) and its waves take every W-th unit of it.
-template <int D, typename S, int DR, bool SITE>
+template <int D, typename S, int DR, bool SITE, bool R12>
 __global__ __launch_bounds__(kBlock) void k_batch_md(const vec2_t<S>* __restrict__ U_in,
                                                      vec2_t<S>* __restrict__ U_out, S* __restrict__ E,
                                                      const ChainDesc* __restrict__ cd,
@@ -51,9 +51,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_md(const vec2_t<S>* __restrict
         vec2_t<S>* uo = DR >= 0 ? U_out + d.u_off : nullptr;
         S* e = E + d.e_off;
         if (SITE)
-            md_site<D, S, S, DR>(ui, uo, e, d.g, s, fs, (S)kick, df);
+            md_site<D, S, S, DR, R12>(ui, uo, e, d.g, s, fs, (S)kick, df);
         else
-            md_link<D, S, S, DR>(ui, uo, e, d.g, mu, s, fs, (S)kick, df);
+            md_link<D, S, S, DR, R12>(ui, uo, e, d.g, mu, s, fs, (S)kick, df);
     }
 }
 
@@ -147,6 +147,42 @@ __global__ __launch_bounds__(kBlock) void k_batch_tile_sums(const vec2_t<S>* __r
     if ((threadIdx.x & 63) == 0)
 #pragma unroll
         for (int k = 0; k < 4; ++k) partials[tile * 4 + k] = acc[k];
+}
+
+// largest su3_distance_link of each tile's links, then of all tiles
+template <int D, typename S>
+__global__ __launch_bounds__(kBlock) void k_batch_su3_distance(const vec2_t<S>* __restrict__ U,
+                                                               const ChainDesc* __restrict__ cd,
+                                                               const int32_t* __restrict__ tile_chain,
+                                                               int64_t ntile, double* __restrict__ partials)
+{
+    const int64_t tile = tile_of_wave();
+    if (tile >= ntile) return;
+    const ChainDesc& d = cd[tile_chain[tile]];
+    const int64_t s = (tile - d.tile0) * kTile + (threadIdx.x & 63);
+    double dev = 0.0;
+    if (s < d.g.nsite)
+#pragma unroll
+        for (int mu = 0; mu < D; ++mu)
+            dev = fmax(dev, su3_distance_link(load_link<double, S>(U + d.u_off, d.g.npad, mu, s)));
+    dev = wave_max(dev);
+    if ((threadIdx.x & 63) == 0) partials[tile * 4] = dev;
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_max_tiles(const double* __restrict__ partials, int64_t ntile,
+                                                            double* __restrict__ out)
+{
+    double m = 0.0;
+    for (int64_t t = threadIdx.x; t < ntile; t += kBlock) m = fmax(m, partials[t * 4]);
+    m = wave_max(m);
+    __shared__ double sh[kBlock / 64];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = sh[0];
+        for (int w = 1; w < kBlock / 64; ++w) r = fmax(r, sh[w]);
+        out[0] = r;
+    }
 }
 
 // block c sums the tiles of chain c in a fixed order
@@ -265,7 +301,7 @@ static int64_t resident_blocks(K kernel, int block)
 
 // launch == false only settles the grid (the occupancy query), so that a
 // later launch under stream capture makes no device query
-template <int D, typename S, int DR, bool SITE>
+template <int D, typename S, int DR, bool SITE, bool R12>
 static hipError_t batch_md(const lqcd_batch* b, int block, bool persistent, const void* U_in, void* U_out,
                            double kick, double dt, bool launch)
 {
@@ -275,12 +311,12 @@ static hipError_t batch_md(const lqcd_batch* b, int block, bool persistent, cons
     if (persistent) {
         static int64_t resident[5] = {-1, -1, -1, -1, -1};  // by block / 64
         int64_t& res = resident[W];
-        if (res < 0) res = resident_blocks(k_batch_md<D, S, DR, SITE>, block);
+        if (res < 0) res = resident_blocks(k_batch_md<D, S, DR, SITE, R12>, block);
         if (res > 0 && res < groups) groups = res;
     }
     if (!launch) return hipSuccess;
     const int64_t chunk = (nunit + groups - 1) / groups;
-    k_batch_md<D, S, DR, SITE><<<(unsigned)groups, block, 0, b->ctx->stream>>>(
+    k_batch_md<D, S, DR, SITE, R12><<<(unsigned)groups, block, 0, b->ctx->stream>>>(
         (const vec2_t<S>*)U_in, (vec2_t<S>*)U_out, (S*)b->E, b->d_chains, b->d_tile_chain, nunit, chunk,
         kick, dt);
     return hipGetLastError();
@@ -291,8 +327,16 @@ static hipError_t batch_md_shape(const lqcd_batch* b, int config, const void* U_
                                  double dt, bool launch)
 {
     const BatchShape sh = batch_shape(config);
-    if (sh.site) return batch_md<D, S, DR, true>(b, sh.block, sh.persistent, U_in, U_out, kick, dt, launch);
-    return batch_md<D, S, DR, false>(b, sh.block, sh.persistent, U_in, U_out, kick, dt, launch);
+    // links read as two rows: f32 exp-drift integrations of SU(3) links
+    if constexpr (sizeof(S) == 4 && DR != DRIFT_ADDITIVE) {
+        if (b->md_r12) {
+            if (sh.site)
+                return batch_md<D, S, DR, true, true>(b, sh.block, sh.persistent, U_in, U_out, kick, dt, launch);
+            return batch_md<D, S, DR, false, true>(b, sh.block, sh.persistent, U_in, U_out, kick, dt, launch);
+        }
+    }
+    if (sh.site) return batch_md<D, S, DR, true, false>(b, sh.block, sh.persistent, U_in, U_out, kick, dt, launch);
+    return batch_md<D, S, DR, false, false>(b, sh.block, sh.persistent, U_in, U_out, kick, dt, launch);
 }
 
 static hipError_t batch_md_op(const lqcd_batch* b, int config, const void* U_in, void* U_out, double kick,
@@ -368,6 +412,14 @@ hipError_t launch_batch_sums(const lqcd_batch* b, const void* U, const void* E, 
     prof_end(b->ctx, &ev);
     k_batch_chain_sums<<<(unsigned)b->chains.size(), kBlock, 0, b->ctx->stream>>>(b->d_partials, b->d_chains,
                                                                                  d_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_su3_distance(const lqcd_batch* b, const void* U, double* d_out1)
+{
+    BATCH_DISPATCH(b, k_batch_su3_distance<D, S><<<tile_grid(b), kBlock, 0, b->ctx->stream>>>(
+                          (const vec2_t<S>*)U, b->d_chains, b->d_tile_chain, b->ntile, b->d_partials));
+    k_batch_max_tiles<<<1, kBlock, 0, b->ctx->stream>>>(b->d_partials, b->ntile, d_out1);
     return hipGetLastError();
 }
 

@@ -64,8 +64,7 @@ __global__ __launch_bounds__(kBlock) void k_final_reduce(const double* __restric
 // max over lanes of a wave, then over the waves of the block (thread 0 gets it)
 __device__ __forceinline__ double block_max(double v)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    v = wave_max(v);
     __shared__ double sh[kBlock / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0) sh[wave] = v;
@@ -76,9 +75,8 @@ __device__ __forceinline__ double block_max(double v)
     return m;
 }
 
-// Distance of the stored links from SU(3): the largest of max_ij |(U U^+ - 1)_ij|
-// and |det U - 1| over all links, in double arithmetic (a NaN entry gives
-// +inf).  Per-block maxima first, then one block over them.
+// largest su3_distance_link over the lattice: per-block maxima, then one
+// block over them
 template <int D, typename S>
 __global__ __launch_bounds__(kBlock) void k_su3_distance_partials(const vec2_t<S>* __restrict__ U, Geom g,
                                                                   double* __restrict__ partials)
@@ -87,22 +85,7 @@ __global__ __launch_bounds__(kBlock) void k_su3_distance_partials(const vec2_t<S
     const int64_t step = (int64_t)gridDim.x * kBlock;
     for (int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x; s < g.nsite; s += step)
 #pragma unroll
-        for (int mu = 0; mu < D; ++mu) {
-            const mat3<double> u = load_link<double, S>(U, g.npad, mu, s);
-            mat3<double> p;
-            mul<false, true>(u, u, p);
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    const double re = p.e[i * 3 + j].re - (i == j ? 1.0 : 0.0), im = p.e[i * 3 + j].im;
-                    const double d = sqrt(re * re + im * im);
-                    dev = d == d ? fmax(dev, d) : INFINITY;
-                }
-            const cplx<double> dt = det(u);
-            const double d = sqrt((dt.re - 1.0) * (dt.re - 1.0) + dt.im * dt.im);
-            dev = d == d ? fmax(dev, d) : INFINITY;
-        }
+        for (int mu = 0; mu < D; ++mu) dev = fmax(dev, su3_distance_link(load_link<double, S>(U, g.npad, mu, s)));
     const double m = block_max(dev);
     if (threadIdx.x == 0) partials[(int64_t)blockIdx.x * kReduceSlots] = m;
 }

@@ -54,6 +54,34 @@ __device__ __forceinline__ double efield_energy_link(const S* __restrict__ E, co
     return t / 2.0;
 }
 
+// distance of one link from SU(3): max(max_ij |(U U^+ - 1)_ij|, |det U - 1|)
+// in double arithmetic; NaN entries give +inf
+__device__ __forceinline__ double su3_distance_link(const mat3<double>& u)
+{
+    mat3<double> p;
+    mul<false, true>(u, u, p);
+    double dev = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double re = p.e[i * 3 + j].re - (i == j ? 1.0 : 0.0), im = p.e[i * 3 + j].im;
+            const double d = sqrt(re * re + im * im);
+            dev = d == d ? fmax(dev, d) : INFINITY;
+        }
+    const cplx<double> dt = det(u);
+    const double d = sqrt((dt.re - 1.0) * (dt.re - 1.0) + dt.im * dt.im);
+    return d == d ? fmax(dev, d) : INFINITY;
+}
+
+// butterfly max over the 64 lanes of a wave
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
 // butterfly sum over the 64 lanes of a wave; every lane gets the total
 template <int K>
 __device__ __forceinline__ void wave_sum(double (&v)[K])

@@ -805,6 +805,14 @@ class BatchedChains:
     def set_graph(self, enable: bool) -> None:
         _raise(F.lib().lqcd_batch_graph_enable(self.handle, int(bool(enable))))
 
+    def set_md_reconstruct(self, mode: int) -> None:
+        _raise(F.lib().lqcd_batch_reconstruct_set(self.handle, mode))
+
+    def md_reconstruct_last(self) -> bool:
+        mode, used = C.c_int(), C.c_int()
+        _raise(F.lib().lqcd_batch_reconstruct_get(self.handle, C.byref(mode), C.byref(used)))
+        return bool(used.value)
+
 
 class HeatBathSweep:
     """``HeatBathSweep`` (src/simulation/monte_carlo/heat_bath.rs:20-123) on the

@@ -135,6 +135,8 @@ SIGNATURES = {
     "lqcd_magnetic_field_vec": (_I, [_P, _P, _S]),
     "lqcd_field_strength_device": (_I, [_P, _I, _P, _S]),
     "lqcd_md_reconstruct_set": (_I, [_P, _I]),
+    "lqcd_batch_reconstruct_set": (_I, [_P, _I]),
+    "lqcd_batch_reconstruct_get": (_I, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "lqcd_md_reconstruct_get": (_I, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "lqcd_su3_distance": (_I, [_P, C.POINTER(C.c_double)]),
     "lqcd_state_write": (_I, [_P, _D, C.c_char_p, _I]),
