@@ -94,14 +94,10 @@ __device__ __forceinline__ mat3<T> load_link(const vec2_t<S>* __restrict__ U, in
     return m;
 }
 
-// U_mu(x) from its first two rows: for a matrix of SU(3) the third row is
-// the complex conjugate of the cross product of the first two,
-//   u_2c = conj(u_0,c+1 u_1,c+2 - u_0,c+2 u_1,c+1),
-// so six of the nine planes are read.  Exact for SU(3); callers use it only on
-// links checked to be in SU(3) to rounding.
+// The first two rows of U_mu(x) (planes 0-5); row 2 is not written.
 template <typename T, typename S>
-__device__ __forceinline__ mat3<T> load_link_r12(const vec2_t<S>* __restrict__ U, int64_t npad, int mu,
-                                                 int64_t s)
+__device__ __forceinline__ mat3<T> load_link_rows01(const vec2_t<S>* __restrict__ U, int64_t npad, int mu,
+                                                    int64_t s)
 {
     using E = vec2_t<S>;
     const __amdgpu_buffer_rsrc_t r = planes_rsrc(U + (int64_t)mu * 9 * npad, npad, 9);
@@ -113,14 +109,20 @@ __device__ __forceinline__ mat3<T> load_link_r12(const vec2_t<S>* __restrict__ U
         const E v = buf_load<E>(r, off, (uint32_t)k * plane);
         m.e[k] = {T(v.x), T(v.y)};
     }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
-        const cplx<T> a = m.e[c1], b = m.e[3 + c2], d = m.e[c2], f = m.e[3 + c1];
-        const T re = fma(a.re, b.re, -a.im * b.im) - fma(d.re, f.re, -d.im * f.im);
-        const T im = fma(a.re, b.im, a.im * b.re) - fma(d.re, f.im, d.im * f.re);
-        m.e[6 + c] = {re, -im};
-    }
+    return m;
+}
+
+// U_mu(x) from its first two rows: for a matrix of SU(3) the third row is
+// the complex conjugate of the cross product of the first two,
+//   u_2c = conj(u_0,c+1 u_1,c+2 - u_0,c+2 u_1,c+1),
+// so six of the nine planes are read.  Exact for SU(3); callers use it only on
+// links checked to be in SU(3) to rounding.
+template <typename T, typename S>
+__device__ __forceinline__ mat3<T> load_link_r12(const vec2_t<S>* __restrict__ U, int64_t npad, int mu,
+                                                 int64_t s)
+{
+    mat3<T> m = load_link_rows01<T, S>(U, npad, mu, s);
+    complete_su3(m);
     return m;
 }
 

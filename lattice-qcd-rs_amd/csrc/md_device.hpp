@@ -39,6 +39,34 @@ __device__ __forceinline__ void staple_sum(const vec2_t<S>* __restrict__ U, cons
 {
     set_zero(sigma);
     const int64_t s_pmu = s + nb.up[mu];
+    if constexpr (R12) {
+        // links and staples in SU(3): products as rows 0, 1 (mul_rows01),
+        // the staple's row 2 rebuilt into Sigma (add_su3)
+#pragma unroll
+        for (int nu = 0; nu < D; ++nu) {
+            if (nu == mu) continue;
+            {
+                const mat3<T> a = load_link_rows01<T, S>(U, g.npad, nu, s_pmu);
+                const mat3<T> b = load_link_r12<T, S>(U, g.npad, mu, s + nb.up[nu]);
+                mat3<T> ab, st;
+                mul_rows01<false, true>(a, b, ab);
+                const mat3<T> c = load_link_r12<T, S>(U, g.npad, nu, s);
+                mul_rows01<false, true>(ab, c, st);
+                add_su3(st, sigma);
+            }
+            {
+                const int64_t s_mnu = s + nb.dn[nu];
+                const mat3<T> a = load_link_r12<T, S>(U, g.npad, nu, s_pmu + nb.dn[nu]);
+                const mat3<T> b = load_link_r12<T, S>(U, g.npad, mu, s_mnu);
+                mat3<T> ab, st;
+                mul_rows01<true, true>(a, b, ab);
+                const mat3<T> c = load_link_r12<T, S>(U, g.npad, nu, s_mnu);
+                mul_rows01<false, false>(ab, c, st);
+                add_su3(st, sigma);
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int nu = 0; nu < D; ++nu) {
         if (nu == mu) continue;
@@ -74,6 +102,49 @@ __device__ __forceinline__ void staple_sums_site(const vec2_t<S>* __restrict__ U
 {
 #pragma unroll
     for (int mu = 0; mu < D; ++mu) set_zero(sig[mu]);
+    if constexpr (R12) {
+        // links, W and staples in SU(3): products as rows 0, 1 (mul_rows01);
+        // W gets its row 2 rebuilt (it is used as W^+), the staples theirs
+        // inside the accumulation (add_su3)
+#pragma unroll
+        for (int mu = 0; mu < D; ++mu)
+#pragma unroll
+            for (int nu = mu + 1; nu < D; ++nu) {
+                mat3<T> w, st;
+                {
+                    const mat3<T> a = load_link_rows01<T, S>(U, g.npad, nu, s + nb.up[mu]);
+                    const mat3<T> b = load_link_r12<T, S>(U, g.npad, mu, s + nb.up[nu]);
+                    mul_rows01<false, true>(a, b, w);
+                    complete_su3(w);
+                }
+                {
+                    const mat3<T> c = load_link_r12<T, S>(U, g.npad, nu, s);
+                    mul_rows01<false, true>(w, c, st);
+                    add_su3(st, sig[mu]);
+                }
+                {
+                    const mat3<T> c = load_link_r12<T, S>(U, g.npad, mu, s);
+                    mul_rows01<true, true>(w, c, st);
+                    add_su3(st, sig[nu]);
+                }
+            }
+#pragma unroll
+        for (int nu = 0; nu < D; ++nu) {
+            const int64_t s_mnu = s + nb.dn[nu];
+            const mat3<T> c = load_link_r12<T, S>(U, g.npad, nu, s_mnu);
+#pragma unroll
+            for (int mu = 0; mu < D; ++mu) {
+                if (mu == nu) continue;
+                const mat3<T> a = load_link_r12<T, S>(U, g.npad, nu, s_mnu + nb.up[mu]);
+                const mat3<T> b = load_link_r12<T, S>(U, g.npad, mu, s_mnu);
+                mat3<T> ab, st;
+                mul_rows01<true, true>(a, b, ab);
+                mul_rows01<false, false>(ab, c, st);
+                add_su3(st, sig[mu]);
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int mu = 0; mu < D; ++mu)
 #pragma unroll

@@ -83,6 +83,70 @@ __device__ __forceinline__ void mul(const mat3<T>& a, const mat3<T>& b, mat3<T>&
         }
 }
 
+// rows 0 and 1 of c = op(a) * op(b), in the accumulation order of mul; row 2
+// of c is not written.  Needs rows 0, 1 of op(a) and all of op(b).
+template <bool DA, bool DB, typename T>
+__device__ __forceinline__ void mul_rows01(const mat3<T>& a, const mat3<T>& b, mat3<T>& c)
+{
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            T re = T(0), im = T(0);
+#pragma unroll
+            for (int kk = 1; kk < 3; ++kk) {
+                const int k = (i + kk) % 3;
+                const cplx<T> x = elem<DA>(a, i, k);
+                const cplx<T> y = elem<DB>(b, k, j);
+                re = fma(x.re, y.re, re);
+                re = fma(-x.im, y.im, re);
+                im = fma(x.re, y.im, im);
+                im = fma(x.im, y.re, im);
+            }
+            const cplx<T> x = elem<DA>(a, i, i);
+            const cplx<T> y = elem<DB>(b, i, j);
+            re = fma(-x.im, y.im, re);
+            im = fma(x.re, y.im, im);
+            im = fma(x.im, y.re, im);
+            re = fma(x.re, y.re, re);
+            c.e[i * 3 + j] = {re, im};
+        }
+}
+
+// Entry (2, c) of a matrix of SU(3) from its rows 0 and 1:
+// conj(m_0,c+1 m_1,c+2 - m_0,c+2 m_1,c+1), added to acc, with one rounding
+// per part and the leading product added last.
+template <typename T>
+__device__ __forceinline__ cplx<T> su3_row2_add(const mat3<T>& m, int c, cplx<T> acc)
+{
+    const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+    const cplx<T> a = m.e[c1], b = m.e[3 + c2], d = m.e[c2], f = m.e[3 + c1];
+    const T re = fma(a.re, b.re, fma(-a.im, b.im, fma(-d.re, f.re, fma(d.im, f.im, acc.re))));
+    const T nim = fma(d.re, f.im, fma(d.im, f.re, fma(-a.re, b.im, fma(-a.im, b.re, acc.im))));
+    return {re, nim};
+}
+
+// row 2 of m in SU(3) from its rows 0 and 1
+template <typename T>
+__device__ __forceinline__ void complete_su3(mat3<T>& m)
+{
+    const cplx<T> z = {T(0), T(0)};
+    const cplx<T> r0 = su3_row2_add(m, 0, z), r1 = su3_row2_add(m, 1, z), r2 = su3_row2_add(m, 2, z);
+    m.e[6] = r0;
+    m.e[7] = r1;
+    m.e[8] = r2;
+}
+
+// acc += m for m in SU(3) given by its rows 0 and 1
+template <typename T>
+__device__ __forceinline__ void add_su3(const mat3<T>& m, mat3<T>& acc)
+{
+#pragma unroll
+    for (int k = 0; k < 6; ++k) acc.e[k] = {acc.e[k].re + m.e[k].re, acc.e[k].im + m.e[k].im};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc.e[6 + c] = su3_row2_add(m, c, acc.e[6 + c]);
+}
+
 // c += op(a) * op(b)
 template <bool DA, bool DB, typename T>
 __device__ __forceinline__ void mul_add(const mat3<T>& a, const mat3<T>& b, mat3<T>& c)
