@@ -656,8 +656,10 @@ def main():
                               ("reunitarize", F.KERNEL_REUNIT, REALS_REUNIT),
                               ("momenta", F.KERNEL_MOMENTA, REALS_MOMENTA)]:
         reals32 = {"kick_drift": reals_kd, "kick": REALS_KICK_R12 if r12 else REALS_KICK}.get(name, reals)
+        reals64 = {"kick_drift": REALS_KICK_DRIFT_R12 if f64["r12"] else REALS_KICK_DRIFT,
+                   "kick": REALS_KICK_R12 if f64["r12"] else REALS_KICK}.get(name, reals)
         kernels["f32_" + name] = kernel_line(mixed, kind, reals32, 4)
-        kernels["f64_" + name] = kernel_line(f64, kind, reals, 8)
+        kernels["f64_" + name] = kernel_line(f64, kind, reals64, 8)
 
     cpu = cpu_baseline(args) if args.cpu_md_steps > 0 else None
     copy_gbs = stream_copy_gbs(lqcd)

@@ -294,12 +294,13 @@ int lqcd_normalize_link_matrices(lqcd_lattice* lat);
 int lqcd_md_autotune(lqcd_lattice* lat, int drift, int* variant);
 int lqcd_md_variant_set(lqcd_lattice* lat, int variant);
 int lqcd_md_variant_get(const lqcd_lattice* lat, int* variant);
-/* Two-row link reads in f32 exp-drift integrations.  No counterpart in the
- * reference.  With mode 1 (default) an integration of an f32 lattice with
- * LQCD_DRIFT_EXP first measures the distance of the links from SU(3)
- * (lqcd_su3_distance); when it is at most 64 f32 ulps of 1 the MD kernels
- * read only the first two rows of every link and rebuild the third as
- * conj(row0 x row1), exact on SU(3).  Mode 0 reads all nine entries always.
+/* Two-row link reads in exp-drift integrations.  No counterpart in the
+ * reference.  With mode 1 (default) an integration with LQCD_DRIFT_EXP first
+ * measures the distance of the links from SU(3) (lqcd_su3_distance); when it
+ * is at most 64 ulps of 1 in the storage type (3.8e-6 for f32, 7.1e-15 for
+ * f64) the MD kernels read only the first two rows of every link and rebuild
+ * the third as conj(row0 x row1), exact on SU(3); products of SU(3) factors
+ * are formed the same way.  Mode 0 reads all nine entries always.
  * *last_used tells whether the last integration read two rows. */
 int lqcd_md_reconstruct_set(lqcd_lattice* lat, int mode);
 int lqcd_md_reconstruct_get(const lqcd_lattice* lat, int* mode, int* last_used);
@@ -375,9 +376,9 @@ int lqcd_batch_config_set(lqcd_batch* batch, int config);
 int lqcd_batch_config_get(const lqcd_batch* batch, int* config);
 /* 1 (default): replay a captured graph per trajectory; 0: launch eagerly */
 int lqcd_batch_graph_enable(lqcd_batch* batch, int enable);
-/* as lqcd_md_reconstruct_set / _get for the chains: lqcd_batch_hmc with f32,
+/* as lqcd_md_reconstruct_set / _get for the chains: lqcd_batch_hmc with
  * LQCD_DRIFT_EXP and reunitarize set reads the links as two rows when all
- * chains' links are within 64 f32 ulps of SU(3).  No counterpart in the
+ * chains' links are within 64 ulps of SU(3).  No counterpart in the
  * reference. */
 int lqcd_batch_reconstruct_set(lqcd_batch* batch, int mode);
 int lqcd_batch_reconstruct_get(const lqcd_batch* batch, int* mode, int* last_used);

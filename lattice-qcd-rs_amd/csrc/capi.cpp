@@ -117,19 +117,19 @@ struct R12Scope {
 };
 
 // n steps of the chosen scheme starting from U[cur]; *last gets the buffer
-// holding the final links.  U[cur] is left untouched.  An f32 integration
-// with the exp drift keeps the links in SU(3) (up to rounding), so when the
+// holding the final links.  U[cur] is left untouched.  An integration with
+// the exp drift keeps the links in SU(3) (up to rounding), so when the
 // starting links are in SU(3) its kernels read only their first two rows.
 int integrate_steps(lqcd_lattice* L, double dt, size_t n, int drift, int scheme, int* last)
 {
     R12Scope scope{L};
     L->md_r12 = false;
-    if (L->precision == LQCD_F32 && drift == LQCD_DRIFT_EXP && L->md_r12_mode) {
+    if (drift == LQCD_DRIFT_EXP && L->md_r12_mode) {
         HIP_TRY(launch_su3_distance(L, L->U[L->cur], L->result + 8));
         HIP_TRY(hipMemcpyAsync(L->host_result + 8, L->result + 8, sizeof(double), hipMemcpyDeviceToHost,
                                L->ctx->stream));
         HIP_TRY(hipStreamSynchronize(L->ctx->stream));
-        L->md_r12 = L->host_result[8] <= kR12Distance;
+        L->md_r12 = L->host_result[8] <= r12_distance(L->precision);
     }
     L->md_r12_last = L->md_r12;
     if (int rc = ensure_md_variant(L, drift)) return rc;

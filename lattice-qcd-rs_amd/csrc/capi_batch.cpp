@@ -427,14 +427,14 @@ int lqcd_batch_hmc(lqcd_batch* b, double dt, size_t n_steps, int drift, int sche
     CHECK_ARG(scheme == LQCD_SCHEME_REFERENCE || scheme == LQCD_SCHEME_LEAPFROG, "unknown scheme");
     if (n_traj == 0) return LQCD_OK;
     DeviceGuard dg(b->ctx->device);
-    // two-row link reads: f32, exp drift, reunitarised trajectories (every
+    // two-row link reads: exp drift, reunitarised trajectories (every
     // trajectory then starts from links in SU(3)) and links in SU(3) now
     b->md_r12 = false;
-    if (b->precision == LQCD_F32 && drift == LQCD_DRIFT_EXP && reunitarize && b->md_r12_mode) {
+    if (drift == LQCD_DRIFT_EXP && reunitarize && b->md_r12_mode) {
         HIP_TRY(launch_batch_su3_distance(b, b->U[0], b->d_scratch));
         HIP_TRY(hipMemcpyAsync(b->h_scratch, b->d_scratch, sizeof(double), hipMemcpyDeviceToHost, b->ctx->stream));
         HIP_TRY(hipStreamSynchronize(b->ctx->stream));
-        b->md_r12 = b->h_scratch[0] <= kR12Distance;
+        b->md_r12 = b->h_scratch[0] <= r12_distance(b->precision);
     }
     if (int rc = ensure_config(b, drift)) return rc;
     const size_t nc = b->chains.size();

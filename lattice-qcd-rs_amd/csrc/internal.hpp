@@ -42,7 +42,7 @@ struct lqcd_lattice {
     uint32_t* mh_acc = nullptr;    // per-link Metropolis accept flags
     int nblocks_reduce = 0;
     int md_variant = -1;           // work decomposition of the MD kernels (-1: not tuned)
-    int md_r12_mode = 1;           // 1: f32 exp-drift integrations read SU(3) links as two rows
+    int md_r12_mode = 1;           // 1: exp-drift integrations read SU(3) links as two rows
     bool md_r12 = false;           // set for the duration of one integration that does
     bool md_r12_last = false;      // whether the last integration did
     void* gauss_g = nullptr;       // Gauss law G(x): 9 planes of double2
@@ -58,9 +58,12 @@ constexpr int kMaxReduceBlocks = 4096;
 // heat-bath blocks (32768..)
 constexpr size_t kMaxMhUpdates = 4000;
 // Links whose distance from SU(3) (launch_su3_distance) is at most this are
-// read as two rows by f32 exp-drift integrations: 64 f32 ulps of 1, i.e. the
-// rounding of a stored SU(3) matrix with room for the growth of a trajectory.
-constexpr double kR12Distance = 64.0 * 5.9604644775390625e-8;
+// read as two rows by exp-drift integrations: 64 ulps of 1 in the storage
+// type, i.e. the rounding of a stored SU(3) matrix with room for the growth
+// of a trajectory.
+constexpr double kR12DistanceF32 = 64.0 * 5.9604644775390625e-8;
+constexpr double kR12DistanceF64 = 64.0 * 1.1102230246251565e-16;
+inline double r12_distance(int precision) { return precision == 1 ? kR12DistanceF32 : kR12DistanceF64; }
 
 enum ProfileKind {
     PROF_KICK_DRIFT = 0,

@@ -327,8 +327,8 @@ static hipError_t batch_md_shape(const lqcd_batch* b, int config, const void* U_
                                  double dt, bool launch)
 {
     const BatchShape sh = batch_shape(config);
-    // links read as two rows: f32 exp-drift integrations of SU(3) links
-    if constexpr (sizeof(S) == 4 && DR != DRIFT_ADDITIVE) {
+    // links read as two rows: exp-drift integrations of SU(3) links
+    if constexpr (DR != DRIFT_ADDITIVE) {
         if (b->md_r12) {
             if (sh.site)
                 return batch_md<D, S, DR, true, true>(b, sh.block, sh.persistent, U_in, U_out, kick, dt, launch);

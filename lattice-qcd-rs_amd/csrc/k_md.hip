@@ -140,13 +140,13 @@ static void launch_md_variant_r(const lqcd_lattice* L, int variant, const void* 
     }
 }
 
-// links read as two rows (R12) only for f32 storage, when the integration
+// links read as two rows (R12) when the integration uses the exp drift and
 // has checked them to be in SU(3) (lqcd_lattice::md_r12)
 template <int D, typename S, int DR>
 static void launch_md_variant(const lqcd_lattice* L, int variant, const void* U_in, void* U_out, void* E,
                               double kick, double dfac)
 {
-    if constexpr (sizeof(S) == 4 && DR != DRIFT_ADDITIVE) {
+    if constexpr (DR != DRIFT_ADDITIVE) {
         if (L->md_r12) {
             launch_md_variant_r<D, S, DR, true>(L, variant, U_in, U_out, E, kick, dfac);
             return;

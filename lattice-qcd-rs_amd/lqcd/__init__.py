@@ -225,7 +225,7 @@ class LatticeStateDefault:
         return st
 
     def set_md_reconstruct(self, mode: int) -> None:
-        """1 (default): f32 exp-drift integrations read SU(3) links as two rows;
+        """1 (default): exp-drift integrations read SU(3) links as two rows;
         0: always nine entries."""
         _raise(F.lib().lqcd_md_reconstruct_set(self.handle, mode))
 

@@ -88,9 +88,9 @@ def test_batch_hmc_matches_single_lattices(precision, config, drift, scheme, d):
     site = (config // 3) % 2 == 1
     stats = b.hmc(dt, n_steps, n_traj, drift, scheme, reunitarize=True)
     assert b.counter == n_traj
-    # f32 exp-drift chains of SU(3) links read the links as two rows, as a
+    # exp-drift chains of SU(3) links read the links as two rows, as a
     # lattice of its own does
-    assert b.md_reconstruct_last() == (precision == lqcd.F32 and drift == lqcd.DRIFT_EXP)
+    assert b.md_reconstruct_last() == (drift == lqcd.DRIFT_EXP)
     tol_u = 1e-12 if precision == lqcd.F64 else 2e-5
     for i, ch in enumerate(chains):
         u_ref, ref = single_run(ch, d, precision, 2 if site else 0, dt, n_steps, n_traj, drift,

@@ -1,7 +1,7 @@
-"""Two-row link reads of f32 exp-drift integrations (lqcd_md_reconstruct_*):
-used only when the links are in SU(3) to 64 f32 ulps, agreeing with the
-nine-entry path to f32 rounding and with the f64 oracle to the f32-path
-tolerance; never used for f64, the additive drift or links off SU(3)."""
+"""Two-row link reads of exp-drift integrations (lqcd_md_reconstruct_*): used
+only when the links are in SU(3) to 64 ulps of the storage type, agreeing
+with the nine-entry path to rounding and with the f64 oracle to the tolerance
+of each path; never used for the additive drift or links off SU(3)."""
 import sys
 
 import numpy as np
@@ -65,15 +65,29 @@ def test_two_row_reads_match_nine_entries_and_oracle():
     assert st.su3_distance() <= 64 * 2.0 ** -24
 
 
-@pytest.mark.parametrize("precision,drift", [(lqcd.F64, lqcd.DRIFT_EXP), (lqcd.F32, lqcd.DRIFT_ADDITIVE)])
-def test_not_used_for_f64_or_additive(precision, drift):
-    announce(f"two-row reads off for precision={precision} drift={drift}")
+def test_two_row_reads_f64_match_oracle():
+    announce("f64 exp drift: two-row vs nine-entry reads vs oracle")
+    ol = O.OracleLattice(4, [4, 4, 4, 6])
+    u = ol.hot_start(14)
+    e = ol.random_efield(15, 0.5 / 6.0)
+    used, a, ea, _ = run(u, e, lqcd.F64, lqcd.DRIFT_EXP, 1)
+    assert used
+    used0, b, eb, _ = run(u, e, lqcd.F64, lqcd.DRIFT_EXP, 0)
+    assert not used0
+    assert np.max(np.abs(a - b)) <= 1e-13 and np.max(np.abs(ea - eb)) <= 1e-13
+    u_ref, e_ref = ol.simulate_symplectic_n(u, e, 0.01, 10, O.DRIFT_EXP, O.SCHEME_LEAPFROG)
+    assert np.max(np.abs(a - u_ref)) <= 1e-12 and np.max(np.abs(ea - e_ref)) <= 1e-12
+
+
+@pytest.mark.parametrize("precision", [lqcd.F64, lqcd.F32])
+def test_not_used_for_additive_drift(precision):
+    announce(f"two-row reads off for the additive drift precision={precision}")
     ol = O.OracleLattice(4, [4, 4, 4, 6])
     u = ol.hot_start(10)
     e = ol.random_efield(11, 0.5 / 6.0)
-    used, a, ea, _ = run(u, e, precision, drift, 1)
+    used, a, ea, _ = run(u, e, precision, lqcd.DRIFT_ADDITIVE, 1)
     assert not used
-    _, b, eb, _ = run(u, e, precision, drift, 0)
+    _, b, eb, _ = run(u, e, precision, lqcd.DRIFT_ADDITIVE, 0)
     assert np.array_equal(a, b) and np.array_equal(ea, eb)
 
 
