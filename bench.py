@@ -417,8 +417,10 @@ def bench_mh3d(args, lqcd):
     ctx.synchronize()
     seconds = time.perf_counter() - t0
     nlink = st.lattice().number_of_canonical_links_space()
-    # one class launch: every link read once (staples), its nlink/6 links written, p and flag stored
-    per_launch = nlink * 18 * 8 + nlink // 6 * (18 * 8 + 12)
+    # one class launch: every link read once (staples; 12 of 18 reals when the links are read
+    # as two rows), its nlink/6 links written, p and flag stored
+    r12 = st.md_reconstruct_last()
+    per_launch = nlink * (12 if r12 else 18) * 8 + nlink // 6 * (18 * 8 + 12)
     rate = kernel_rate(ctx, F.KERNEL_METROPOLIS, per_launch)
     ctx.profile_enable(0)
     value = nlink * args.steps / seconds
@@ -445,7 +447,8 @@ def bench_mh3d(args, lqcd):
                           f"proposal and uniform draws included) of the oracle on 32^3, {t:.1f} s")}
     return contract_line(
         args, "link updates/s, D=3 128^3 FP64 Metropolis", value, "link updates/s", args.steps, seconds, "f64",
-        cfg, roofline_from(rate, "k_metropolis<3,double,RNG> (one checkerboard class)"), cpu,
+        cfg, roofline_from(rate, "k_metropolis<3,double,RNG> (one checkerboard class"
+                           + (", staple links read as two rows)" if r12 else ")")), cpu,
         sweeps_per_s=round(args.steps / seconds, 3), prob_replace_mean=mh.prob_replace_mean(),
         number_replace_last=mh.number_replace_last())
 

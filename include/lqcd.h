@@ -301,7 +301,13 @@ int lqcd_md_variant_get(const lqcd_lattice* lat, int* variant);
  * f64) the MD kernels read only the first two rows of every link and rebuild
  * the third as conj(row0 x row1), exact on SU(3); products of SU(3) factors
  * are formed the same way.  Mode 0 reads all nine entries always.
- * *last_used tells whether the last integration read two rows. */
+ * Device-proposal Metropolis sweeps (lqcd_metropolis_sweep_random) read the
+ * staple links the same way: their proposals are in SU(3).  The library
+ * keeps track of what is known (hot / cold start and reunitarisation give
+ * SU(3); the additive drift, heat bath and overrelaxation leave it; uploads
+ * and host proposals make it unknown) and measures again after 200
+ * SU(3)-preserving updates per link.  *last_used tells whether the last
+ * integration or device Metropolis sweep read two rows. */
 int lqcd_md_reconstruct_set(lqcd_lattice* lat, int mode);
 int lqcd_md_reconstruct_get(const lqcd_lattice* lat, int* mode, int* last_used);
 /* max over links of max(max_ij |(U U^+ - 1)_ij|, |det U - 1|), double arithmetic */

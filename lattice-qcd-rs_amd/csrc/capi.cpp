@@ -116,6 +116,29 @@ struct R12Scope {
     ~R12Scope() { L->md_r12 = false; }
 };
 
+// Whether U[cur] is within r12_distance of SU(3) for an operation of
+// `updates` SU(3)-preserving updates per link.  A measurement
+// (launch_su3_distance, one host sync) is made when nothing is known or when
+// the updates since the last one would pass kR12Recheck; links known to be
+// off SU(3) are not measured again until they are replaced.
+int links_su3(lqcd_lattice* L, int64_t updates, bool* ok)
+{
+    *ok = false;
+    if (L->su3_known == -2) return LQCD_OK;
+    if (L->su3_known >= 0 && L->su3_known + updates <= kR12Recheck) {
+        L->su3_known += updates;
+        *ok = true;
+        return LQCD_OK;
+    }
+    HIP_TRY(launch_su3_distance(L, L->U[L->cur], L->result + 8));
+    HIP_TRY(hipMemcpyAsync(L->host_result + 8, L->result + 8, sizeof(double), hipMemcpyDeviceToHost,
+                           L->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    *ok = L->host_result[8] <= r12_distance(L->precision);
+    L->su3_known = *ok ? updates : -2;
+    return LQCD_OK;
+}
+
 // n steps of the chosen scheme starting from U[cur]; *last gets the buffer
 // holding the final links.  U[cur] is left untouched.  An integration with
 // the exp drift keeps the links in SU(3) (up to rounding), so when the
@@ -125,12 +148,12 @@ int integrate_steps(lqcd_lattice* L, double dt, size_t n, int drift, int scheme,
     R12Scope scope{L};
     L->md_r12 = false;
     if (drift == LQCD_DRIFT_EXP && L->md_r12_mode) {
-        HIP_TRY(launch_su3_distance(L, L->U[L->cur], L->result + 8));
-        HIP_TRY(hipMemcpyAsync(L->host_result + 8, L->result + 8, sizeof(double), hipMemcpyDeviceToHost,
-                               L->ctx->stream));
-        HIP_TRY(hipStreamSynchronize(L->ctx->stream));
-        L->md_r12 = L->host_result[8] <= r12_distance(L->precision);
+        bool ok;
+        if (int rc = links_su3(L, (int64_t)n, &ok)) return rc;
+        L->md_r12 = ok;
     }
+    // the additive drift takes the links off SU(3)
+    if (drift == LQCD_DRIFT_ADDITIVE) L->su3_known = -2;
     L->md_r12_last = L->md_r12;
     if (int rc = ensure_md_variant(L, drift)) return rc;
     int w1, w2;
@@ -363,6 +386,7 @@ int lqcd_links_upload(lqcd_lattice* L, const double* links, size_t n_links)
                            L->ctx->stream));
     HIP_TRY(launch_links_from_aos(L, (const double*)st.p, L->U[L->cur]));
     HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    L->su3_known = -1;
     return LQCD_OK;
 }
 
@@ -386,6 +410,7 @@ int lqcd_links_cold(lqcd_lattice* L)
     DeviceGuard dg(L->ctx->device);
     HIP_TRY(launch_links_cold(L, L->U[L->cur]));
     HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    L->su3_known = 0;
     return LQCD_OK;
 }
 
@@ -395,6 +420,7 @@ int lqcd_links_random(lqcd_lattice* L, uint64_t seed)
     DeviceGuard dg(L->ctx->device);
     HIP_TRY(launch_links_random(L, L->U[L->cur], seed));
     HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    L->su3_known = 0;
     return LQCD_OK;
 }
 
@@ -645,6 +671,10 @@ int lqcd_integrate_link(lqcd_lattice* L, double dt, int drift)
     HIP_TRY(launch_drift(L, L->U[L->cur], L->U[w1], L->E, dt, drift));
     HIP_TRY(hipStreamSynchronize(L->ctx->stream));
     L->cur = w1;
+    if (drift == LQCD_DRIFT_ADDITIVE)
+        L->su3_known = -2;
+    else if (L->su3_known >= 0)
+        ++L->su3_known;
     return LQCD_OK;
 }
 
@@ -742,6 +772,15 @@ int lqcd_metropolis_sweep_random(lqcd_lattice* L, double beta, size_t n_update, 
     CHECK_ARG(n_update <= kMaxMhUpdates, "n_update exceeds the device proposal stream");
     DeviceGuard dg(L->ctx->device);
     if (int rc = metropolis_prepare(L, beta, n_update)) return rc;
+    // the device proposals are orthonormalised (in SU(3)): links in SU(3)
+    // stay there, and the staples read them as two rows
+    R12Scope scope{L};
+    if (L->md_r12_mode) {
+        bool ok;
+        if (int rc = links_su3(L, (int64_t)n_update, &ok)) return rc;
+        L->md_r12 = ok;
+    }
+    L->md_r12_last = L->md_r12;
     for (int mu = 0; mu < L->g.dim; ++mu)
         for (int par = 0; par < 2; ++par)
             HIP_TRY(launch_metropolis_class_rng(L, L->U[L->cur], mu, par, beta, n_update, spread, seed,
@@ -766,6 +805,8 @@ int lqcd_metropolis_sweep(lqcd_lattice* L, double beta, size_t n_update, const d
         for (int par = 0; par < 2; ++par)
             HIP_TRY(launch_metropolis_class(L, L->U[L->cur], mu, par, beta, n_update,
                                             (const double*)dp.p, (const uint64_t*)du.p));
+    // the caller's proposals need not be in SU(3)
+    L->su3_known = -1;
     return metropolis_finish(L, stats);
 }
 
@@ -779,6 +820,8 @@ int lqcd_heat_bath_sweep(lqcd_lattice* L, double beta, uint64_t seed, uint64_t c
     for (int mu = 0; mu < L->g.dim; ++mu)
         for (int par = 0; par < 2; ++par)
             HIP_TRY(launch_heat_bath_class(L, L->U[L->cur], mu, par, beta, seed, counter));
+    // the subgroup factors follow the reference's sigma_3 = identity: off SU(3)
+    L->su3_known = -2;
     HIP_TRY(launch_mh_stats(L, L->result));
     if (int rc = read_results(L, 2)) return rc;
     if (exhausted) *exhausted = (size_t)L->host_result[1];
@@ -794,6 +837,8 @@ static int overrelaxation(lqcd_lattice* L, bool reverse)
     for (int mu = 0; mu < L->g.dim; ++mu)
         for (int par = 0; par < 2; ++par) HIP_TRY(launch_overrelax_class(L, L->U[L->cur], mu, par, reverse));
     HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    // unitary, but det U is not kept at 1
+    L->su3_known = -2;
     return LQCD_OK;
 }
 
@@ -807,6 +852,7 @@ int lqcd_normalize_link_matrices(lqcd_lattice* L)
     DeviceGuard dg(L->ctx->device);
     HIP_TRY(launch_reunitarize(L, L->U[L->cur]));
     HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    L->su3_known = 0;
     return LQCD_OK;
 }
 

@@ -38,13 +38,17 @@ struct lqcd_lattice {
     double* partials = nullptr;  // per-block reduction scratch
     double* result = nullptr;    // device reduction results
     double* host_result = nullptr; // pinned mirror of result
-    double* mh_p = nullptr;        // per-link Metropolis probabilities
-    uint32_t* mh_acc = nullptr;    // per-link Metropolis accept flags
+    double* mh_p = nullptr;        // per-link Metropolis probabilities (class order)
+    uint32_t* mh_acc = nullptr;    // per-link Metropolis accept flags (class order)
     int nblocks_reduce = 0;
     int md_variant = -1;           // work decomposition of the MD kernels (-1: not tuned)
     int md_r12_mode = 1;           // 1: exp-drift integrations read SU(3) links as two rows
     bool md_r12 = false;           // set for the duration of one integration that does
     bool md_r12_last = false;      // whether the last integration did
+    // what is known of the distance of U[cur] from SU(3): >= 0 within
+    // r12_distance, counting the SU(3)-preserving updates since that was
+    // established; -1 unknown; -2 known to be off SU(3)
+    int64_t su3_known = -1;
     void* gauss_g = nullptr;       // Gauss law G(x): 9 planes of double2
     double* gauss_e[2] = {nullptr, nullptr};  // double copies of E for the projection
 };
@@ -64,6 +68,9 @@ constexpr size_t kMaxMhUpdates = 4000;
 constexpr double kR12DistanceF32 = 64.0 * 5.9604644775390625e-8;
 constexpr double kR12DistanceF64 = 64.0 * 1.1102230246251565e-16;
 inline double r12_distance(int precision) { return precision == 1 ? kR12DistanceF32 : kR12DistanceF64; }
+// SU(3)-preserving updates (MD steps, Metropolis updates per link) after
+// which the distance is measured again rather than assumed
+constexpr int64_t kR12Recheck = 200;
 
 enum ProfileKind {
     PROF_KICK_DRIFT = 0,

@@ -86,7 +86,8 @@ __device__ __forceinline__ uint64_t device_accept_word(uint64_t l, uint64_t seed
 
 // staple A_mu(x) of monte_carlo/mod.rs:339-362 in double arithmetic:
 // sum_{nu != mu} U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+ + U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)
-template <int D, typename S>
+// R12: links read as two rows (load_link_r12), for links in SU(3)
+template <int D, typename S, bool R12 = false>
 __device__ __forceinline__ void mc_staple(const vec2_t<S>* __restrict__ U, const Geom& g, int mu, int64_t s,
                                           mat3<double>& a)
 {
@@ -97,20 +98,20 @@ __device__ __forceinline__ void mc_staple(const vec2_t<S>* __restrict__ U, const
     for (int nu = 0; nu < D; ++nu) {
         if (nu == mu) continue;
         {
-            const mat3<double> x = load_link<double, S>(U, g.npad, nu, s_pmu);
-            const mat3<double> y = load_link<double, S>(U, g.npad, mu, s + nb.up[nu]);
+            const mat3<double> x = load_link_sel<R12, double, S>(U, g.npad, nu, s_pmu);
+            const mat3<double> y = load_link_sel<R12, double, S>(U, g.npad, mu, s + nb.up[nu]);
             mat3<double> xy;
             mul<false, true>(x, y, xy);
-            const mat3<double> z = load_link<double, S>(U, g.npad, nu, s);
+            const mat3<double> z = load_link_sel<R12, double, S>(U, g.npad, nu, s);
             mul_add<false, true>(xy, z, a);
         }
         {
             const int64_t s_mnu = s + nb.dn[nu];
-            const mat3<double> x = load_link<double, S>(U, g.npad, nu, s_pmu + nb.dn[nu]);
-            const mat3<double> y = load_link<double, S>(U, g.npad, mu, s_mnu);
+            const mat3<double> x = load_link_sel<R12, double, S>(U, g.npad, nu, s_pmu + nb.dn[nu]);
+            const mat3<double> y = load_link_sel<R12, double, S>(U, g.npad, mu, s_mnu);
             mat3<double> xy;
             mul<true, true>(x, y, xy);
-            const mat3<double> z = load_link<double, S>(U, g.npad, nu, s_mnu);
+            const mat3<double> z = load_link_sel<R12, double, S>(U, g.npad, nu, s_mnu);
             mul_add<false, false>(xy, z, a);
         }
     }
@@ -125,7 +126,7 @@ __device__ __forceinline__ int64_t class_site(const Geom& g, int64_t k, int pari
     return s0 + ((site_parity<D>(g, s0) + parity) & 1);
 }
 
-template <int D, typename S, bool RNG>
+template <int D, typename S, bool RNG, bool R12 = false>
 __global__ __launch_bounds__(kBlock) void k_metropolis(vec2_t<S>* __restrict__ U, Geom g, int mu,
                                                        int parity, double beta, int n_update,
                                                        const double* __restrict__ proposals,
@@ -139,8 +140,8 @@ __global__ __launch_bounds__(kBlock) void k_metropolis(vec2_t<S>* __restrict__ U
     if (k >= half) return;
     const int64_t s = class_site<D>(g, k, parity);
     mat3<double> a;
-    mc_staple<D, S>(U, g, mu, s, a);
-    const mat3<double> u = load_link<double, S>(U, g.npad, mu, s);
+    mc_staple<D, S, R12>(U, g, mu, s, a);
+    const mat3<double> u = load_link_sel<R12, double, S>(U, g.npad, mu, s);
     const int64_t li = s * D + mu;
     const int64_t nlink = g.nsite * D;
     mat3<double> nw = u;
@@ -172,8 +173,11 @@ __global__ __launch_bounds__(kBlock) void k_metropolis(vec2_t<S>* __restrict__ U
         const uint64_t word = RNG ? device_accept_word((uint64_t)li, seed, counter) : uniforms[li];
         accept = word < p_int;
     }
-    p_out[li] = p;
-    acc_out[li] = accept ? 1u : 0u;
+    // per-link outputs in class order (contiguous for the launch); only their
+    // sums are read
+    const int64_t o = (int64_t)(mu * 2 + parity) * half + k;
+    p_out[o] = p;
+    acc_out[o] = accept ? 1u : 0u;
     if (accept) store_link<double, S>(U, g.npad, mu, s, nw);
 }
 
@@ -326,8 +330,9 @@ __global__ __launch_bounds__(kBlock) void k_heat_bath(vec2_t<S>* __restrict__ U,
     mat3<double> nw;
     mul<false, false>(acc, u, nw);
     store_link<double, S>(U, g.npad, mu, s, nw);
-    p_out[li] = 0.0;
-    exhausted[li] = ex;
+    const int64_t o = (int64_t)(mu * 2 + parity) * (g.nsite / 2) + k;
+    p_out[o] = 0.0;
+    exhausted[o] = ex;
 }
 
 // --------------------------------------------------------- overrelaxation
@@ -530,9 +535,14 @@ hipError_t launch_metropolis_class_rng(const lqcd_lattice* L, void* U, int mu, i
     const unsigned grid = grid_for(L->g.nsite / 2);
     lqcd_prof_event ev;
     prof_begin(L->ctx, PROF_METROPOLIS, &ev);
-    LQCD_DISPATCH_DS(L, k_metropolis<D, S, true><<<grid, kBlock, 0, L->ctx->stream>>>(
-                            (vec2_t<S>*)U, L->g, mu, parity, beta, (int)n_update, nullptr, nullptr,
-                            spread, seed, counter, L->mh_p, L->mh_acc));
+    if (L->md_r12)
+        LQCD_DISPATCH_DS(L, k_metropolis<D, S, true, true><<<grid, kBlock, 0, L->ctx->stream>>>(
+                                (vec2_t<S>*)U, L->g, mu, parity, beta, (int)n_update, nullptr, nullptr,
+                                spread, seed, counter, L->mh_p, L->mh_acc));
+    else
+        LQCD_DISPATCH_DS(L, k_metropolis<D, S, true, false><<<grid, kBlock, 0, L->ctx->stream>>>(
+                                (vec2_t<S>*)U, L->g, mu, parity, beta, (int)n_update, nullptr, nullptr,
+                                spread, seed, counter, L->mh_p, L->mh_acc));
     prof_end(L->ctx, &ev);
     return hipGetLastError();
 }

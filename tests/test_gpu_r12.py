@@ -110,3 +110,37 @@ def test_not_used_off_su3():
         assert not used
         _, b, eb, _ = run(links, e, lqcd.F32, lqcd.DRIFT_EXP, 0)
         assert np.array_equal(a, b) and np.array_equal(ea, eb)
+
+
+def test_metropolis_two_row_staples():
+    # device-proposal sweeps keep SU(3) links in SU(3) and read the staples
+    # as two rows; nine-entry reads give the same accept decisions and links
+    # to rounding.  After a heat-bath sweep (off SU(3)) they read nine entries
+    # and agree with that mode to rounding.
+    announce("metropolis two-row staples")
+    ol = O.OracleLattice(4, [4, 4, 4, 6])
+    u = ol.hot_start(16)
+    runs = {}
+    for mode in (1, 0):
+        st = lqcd.LatticeStateDefault(lqcd.LatticeCyclic(1.0, [4, 4, 4, 6], 4), 5.5, None, lqcd.F64)
+        st.set_link_matrix(u)
+        st.normalize_link_matrices()
+        st.set_md_reconstruct(mode)
+        mh = lqcd.MetropolisHastingsSweep(1, 0.2, 99)
+        counts = []
+        for _ in range(3):
+            mh.next_element(st)
+            counts.append(mh.number_replace_last())
+        runs[mode] = (st.md_reconstruct_last(), counts, st.link_matrix(), st)
+    assert runs[1][0] and not runs[0][0]
+    assert runs[1][1] == runs[0][1]
+    assert np.max(np.abs(runs[1][2] - runs[0][2])) <= 1e-13
+    out = {}
+    for mode in (1, 0):
+        st = runs[mode][3]
+        lqcd.HeatBathSweep(3).next_element(st)
+        mh = lqcd.MetropolisHastingsSweep(1, 0.2, 5)
+        mh.next_element(st)
+        out[mode] = (st.md_reconstruct_last(), st.link_matrix())
+    assert not out[1][0]
+    assert np.max(np.abs(out[1][1] - out[0][1])) <= 1e-12
