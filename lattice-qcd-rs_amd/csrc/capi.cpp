@@ -66,25 +66,32 @@ int autotune_md(lqcd_lattice* L, int drift, int* chosen)
     int best = VAR_LINK_XCD;
     float best_ms = 1e30f;
     int rc = LQCD_OK;
-    for (int v = 0; v < VAR_COUNT && rc == LQCD_OK; ++v) {
-        const int reps = 3;
-        hipError_t e = launch_kick_drift_variant(L, v, L->U[L->cur], L->U[w1], L->E, 0.0, 1e-3, drift);
-        if (e == hipSuccess) e = hipEventRecord(t0, L->ctx->stream);
-        for (int r = 0; r < reps && e == hipSuccess; ++r)
-            e = launch_kick_drift_variant(L, v, L->U[L->cur], L->U[w1], L->E, 0.0, 1e-3, drift);
-        if (e == hipSuccess) e = hipEventRecord(t1, L->ctx->stream);
-        if (e == hipSuccess) e = hipEventSynchronize(t1);
-        float ms = 0.f;
-        if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
-        if (e != hipSuccess) {
-            rc = fail(LQCD_ERR_DEVICE, std::string("autotune: ") + hipGetErrorString(e));
-            break;
+    // two rounds over the variants, 6 launches each, keeping each variant's
+    // better round: one noisy round does not decide
+    float vms[VAR_COUNT];
+    for (int v = 0; v < VAR_COUNT; ++v) vms[v] = 1e30f;
+    for (int round = 0; round < 2 && rc == LQCD_OK; ++round)
+        for (int v = 0; v < VAR_COUNT && rc == LQCD_OK; ++v) {
+            const int reps = 6;
+            hipError_t e = launch_kick_drift_variant(L, v, L->U[L->cur], L->U[w1], L->E, 0.0, 1e-3, drift);
+            if (e == hipSuccess) e = hipEventRecord(t0, L->ctx->stream);
+            for (int r = 0; r < reps && e == hipSuccess; ++r)
+                e = launch_kick_drift_variant(L, v, L->U[L->cur], L->U[w1], L->E, 0.0, 1e-3, drift);
+            if (e == hipSuccess) e = hipEventRecord(t1, L->ctx->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(t1);
+            float ms = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+            if (e != hipSuccess) {
+                rc = fail(LQCD_ERR_DEVICE, std::string("autotune: ") + hipGetErrorString(e));
+                break;
+            }
+            if (ms < vms[v]) vms[v] = ms;
         }
-        if (ms < best_ms) {
-            best_ms = ms;
+    for (int v = 0; v < VAR_COUNT; ++v)
+        if (vms[v] < best_ms) {
+            best_ms = vms[v];
             best = v;
         }
-    }
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     if (rc != LQCD_OK) return rc;
