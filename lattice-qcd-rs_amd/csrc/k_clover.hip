@@ -2,9 +2,9 @@
 // every site (LinkMatrix::clover / f_mu_nu / magnetic_field_vec,
 // src/field.rs:806-868), in double arithmetic on either storage type.
 //
-// One thread per site.  The output is written straight in the host layout of
-// the reference (18 doubles per matrix, column-major 3x3 complex):
-// [site][i][j] for the clover and F, [site][dir] for B.
+// The output is in the host layout of the reference (18 doubles per matrix,
+// column-major 3x3 complex): [site][i][j] for the clover and F (stores
+// staged through LDS), [site][dir] for B; one thread per site.
 #include "dispatch.hpp"
 #include "fields.hpp"
 
@@ -162,51 +162,52 @@ __device__ __forceinline__ double levi_dij(int d, int i, int j)
     return (i > d) == (j > d) ? 1.0 : -1.0;
 }
 
-// MODE 0: clover; 1: f_mu_nu; 2: magnetic_field_vec.  inv = 1 / (8 a^2).
-//
+// Matrix m of every lane of a wave to out[(site * nslot + slot) * 18], the
+// wave's sites being s0 .. s0 + 63 (sites >= nsite are skipped).  The matrices
+// pass through LDS so that each store instruction writes runs of whole
+// 144-byte matrices of consecutive sites (about seven per instruction) instead
+// of 64 pieces 144 * nslot bytes apart.  Every thread of the block calls it.
+__device__ __forceinline__ void store_slot_staged(double2* __restrict__ stage, double* __restrict__ out, int64_t s0,
+                                                  int64_t nsite, int nslot, int slot, const mat3<double>& m)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+            stage[lane * 9 + c * 3 + r] = make_double2(m.e[r * 3 + c].re, m.e[r * 3 + c].im);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const int ch = k * 64 + lane;
+        const int sl = ch / 9, piece = ch - sl * 9;
+        const int64_t site = s0 + sl;
+        if (site < nsite) reinterpret_cast<double2*>(out + (site * nslot + slot) * 18)[piece] = stage[ch];
+    }
+    __syncthreads();
+}
+
+// magnetic_field_vec of every site (thread per site).  inv = 1 / (8 a^2).
 // Pairs i < j are computed once: clover(j, i) = clover(i, j)^+ (each leaf of
-// one is the adjoint of a leaf of the other), so F_ji = -F_ij and
-// F_ij = (C - C^+) / (8 a^2).  B_d = sum_{i,j} eps(d,i,j) F_ij / (2i) takes
-// each unordered pair twice with the same sign, i.e. eps(d,i,j) F_ij / i.
-template <int D, typename S, int MODE>
-__global__ __launch_bounds__(kBlock) void k_clover_fields(const vec2_t<S>* __restrict__ U, Geom g,
-                                                          double* __restrict__ out, double inv)
+// one is the adjoint of a leaf of the other), so F_ij = (C - C^+) / (8 a^2),
+// and B_d = sum_{i,j} eps(d,i,j) F_ij / (2i), which takes each unordered
+// pair twice with the same sign, is the sum of eps(d,i,j) F_ij / i.
+template <int D, typename S>
+__global__ __launch_bounds__(kBlock) void k_magnetic_field(const vec2_t<S>* __restrict__ U, Geom g,
+                                                           double* __restrict__ out, double inv)
 {
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (s >= g.nsite) return;
-    const Neighbours<D> nb = neighbours<D>(g, s);
     const mat3<double> zero = {};
-    if (MODE == 2) {
-        for (int d = 0; d < D; ++d) store_host_matrix(out + 18 * (s * D + d), zero);
-    }
-    for (int i = 0; i < D; ++i) {
-        if (MODE == 0) {
-            mat3<double> c;
-            clover_at<S>(U, g, s, i, i, c);
-            store_host_matrix(out + 18 * ((s * D + i) * D + i), c);
-        } else if (MODE == 1) {
-            store_host_matrix(out + 18 * ((s * D + i) * D + i), zero);
-        }
+    const Neighbours<D> nb = neighbours<D>(g, s);
+    for (int d = 0; d < D; ++d) store_host_matrix(out + 18 * (s * D + d), zero);
+    for (int i = 0; i < D; ++i)
         for (int j = i + 1; j < D; ++j) {
-            mat3<double> c, ca;
+            mat3<double> c, ca, f;
             clover_offdiag<D, S>(U, g, s, nb, i, j, c);
             adjoint(c, ca);
-            if (MODE == 0) {
-                store_host_matrix(out + 18 * ((s * D + i) * D + j), c);
-                store_host_matrix(out + 18 * ((s * D + j) * D + i), ca);
-                continue;
-            }
-            mat3<double> f, fm;
 #pragma unroll
-            for (int k = 0; k < 9; ++k) {
-                f.e[k] = {(c.e[k].re - ca.e[k].re) * inv, (c.e[k].im - ca.e[k].im) * inv};
-                fm.e[k] = {-f.e[k].re, -f.e[k].im};
-            }
-            if (MODE == 1) {
-                store_host_matrix(out + 18 * ((s * D + i) * D + j), f);
-                store_host_matrix(out + 18 * ((s * D + j) * D + i), fm);
-                continue;
-            }
+            for (int k = 0; k < 9; ++k) f.e[k] = {(c.e[k].re - ca.e[k].re) * inv, (c.e[k].im - ca.e[k].im) * inv};
             for (int d = 0; d < D; ++d) {
                 if (d == i || d == j) continue;
                 const double sg = levi_dij(d, i, j);
@@ -215,6 +216,47 @@ __global__ __launch_bounds__(kBlock) void k_clover_fields(const vec2_t<S>* __res
                 for (int k = 0; k < 9; ++k) b.e[k] = {sg * f.e[k].im, -sg * f.e[k].re};
                 add_host_matrix(out + 18 * (s * D + d), b);
             }
+        }
+}
+
+// clover (MODE 0) or f_mu_nu (MODE 1) of every site, one thread per site,
+// inv = 1 / (8 a^2) (the D (D + 1) / 2 clovers of a site share their link loads in L1 and L2;
+// a thread per site and pair is 30 % slower).  clover(j, i) = clover(i, j)^+,
+// so each pair i < j is computed once and F_ji = -F_ij with
+// F_ij = (C - C^+) / (8 a^2); F_ii = 0 as clover(i, i) - clover(i, i) is in
+// the reference.
+template <int D, typename S, int MODE>
+__global__ __launch_bounds__(kBlock) void k_clover_fields(const vec2_t<S>* __restrict__ U, Geom g,
+                                                         double* __restrict__ out, double inv)
+{
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    // every thread reaches the staged stores; threads past the lattice
+    // compute nothing and store nothing
+    __shared__ double2 stage_all[kBlock * 9];
+    double2* stage = stage_all + (threadIdx.x >> 6) * 64 * 9;
+    const int64_t s0 = s - (threadIdx.x & 63);
+    const bool active = s < g.nsite;
+    const Neighbours<D> nb = neighbours<D>(g, active ? s : 0);
+    constexpr int NS = D * D;
+    const mat3<double> zero = {};
+    for (int i = 0; i < D; ++i) {
+        mat3<double> dg = zero;
+        if (MODE == 0 && active) clover_at<S>(U, g, s, i, i, dg);
+        store_slot_staged(stage, out, s0, g.nsite, NS, i * D + i, dg);
+        for (int j = i + 1; j < D; ++j) {
+            mat3<double> c = zero, ca;
+            if (active) clover_offdiag<D, S>(U, g, s, nb, i, j, c);
+            adjoint(c, ca);
+            if (MODE == 1) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) {
+                    const cplx<double> f = {(c.e[k].re - ca.e[k].re) * inv, (c.e[k].im - ca.e[k].im) * inv};
+                    c.e[k] = f;
+                    ca.e[k] = {-f.re, -f.im};
+                }
+            }
+            store_slot_staged(stage, out, s0, g.nsite, NS, i * D + j, c);
+            store_slot_staged(stage, out, s0, g.nsite, NS, j * D + i, ca);
         }
     }
 }
@@ -235,7 +277,7 @@ hipError_t launch_clover_fields(const lqcd_lattice* L, int mode, double* d_out)
                                 (const vec2_t<S>*)L->U[L->cur], L->g, d_out, inv));
         break;
     default:
-        LQCD_DISPATCH_DS(L, k_clover_fields<D, S, 2><<<grid, kBlock, 0, L->ctx->stream>>>(
+        LQCD_DISPATCH_DS(L, k_magnetic_field<D, S><<<grid, kBlock, 0, L->ctx->stream>>>(
                                 (const vec2_t<S>*)L->U[L->cur], L->g, d_out, inv));
         break;
     }
