@@ -26,6 +26,8 @@ Other workloads (--workload), one JSON line each, same keys:
            50 leapfrog steps dt = 0.02, plaquette after every trajectory.
   mh3d     D = 3 128^3 FP64 checkerboard Metropolis sweeps (spread 0.1,
            beta = 6.0), proposals and accept draws generated on the device.
+  stats    statistics::mean_and_variance (two-pass) of 2^27 f64 values resident
+           in device memory.
   small    8^4 FP64: plaquette, HMC trajectories of 20 steps dt = 0.01
            (reference kick-drift-kick scheme, additive drift) and Metropolis
            sweeps (spread 0.1) with reunitarisation every 10 sweeps.
@@ -77,7 +79,7 @@ def parse():
     p.add_argument("--copy-only", action="store_true",
                    help="only run the 1 GiB streaming copy (byte-counter calibration)")
     p.add_argument("--workload", default="hmc32",
-                   choices=["hmc32", "batched", "large", "mh3d", "small", "fields", "state"])
+                   choices=["hmc32", "batched", "large", "mh3d", "small", "fields", "state", "stats"])
     p.add_argument("--precision-trajectories", type=int, default=3,
                    help="trajectories of the FP32-mixed against FP64 error table (0 = skip)")
     p.add_argument("--gauss-checks", type=int, default=4000,
@@ -502,6 +504,48 @@ def bench_fields(args, lqcd):
         cfg, roofline_from(rate, "k_clover_fields<4,float,1> (clover, F_mu_nu)"), cpu)
 
 
+def bench_stats(args, lqcd):
+    """statistics::mean_and_variance (statistics/mod.rs:375-399, two passes:
+    the sum, then sum (x - mean)^2) of 2^27 f64 values resident in device
+    memory; each pass is one reduction kernel reading the values once."""
+    from lqcd import statistics as S
+    F = lqcd._ffi
+    n = 1 << 27
+    x = np.random.default_rng(args.seed & 0xFFFFFFFF).uniform(-1.0, 1.0, n)
+    vals = S.DeviceValues.from_numpy(x)
+    ctx = lqcd.Context.default()
+
+    def step():
+        return S.mean_and_variance(vals, ctx)
+
+    for _ in range(args.warmup):
+        step()
+    ctx.profile_enable(1 << F.KERNEL_STATS)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = step()
+    ctx.synchronize()
+    seconds = time.perf_counter() - t0
+    rate = kernel_rate(ctx, F.KERNEL_STATS, 8 * n)
+    ctx.profile_enable(0)
+    cfg = {"workload": "mean_and_variance (two-pass) of 2^27 f64 values in device memory", "values": n}
+    cpu = None
+    if args.cpu_md_steps > 0:
+        import oracle as O
+        k, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < min(args.cpu_seconds, 10.0):
+            O.stats_mean_and_variance(x)
+            k += 1
+        t = time.perf_counter() - t0
+        cpu = {"value": n * k / t, "unit": "values/s", "cores": 1, "kind": "port",
+               "sample": f"{k} two-pass mean_and_variance of the same 2^27 values on the oracle, 1 thread, {t:.1f} s"}
+    return contract_line(
+        args, "statistics values/s, mean_and_variance of 2^27 f64", n * args.steps / seconds, "values/s",
+        args.steps, seconds, "f64", cfg, roofline_from(rate, "k_stats_partials<double> (one pass of the values)"),
+        cpu, result=[float(res[0]), float(res[1])])
+
+
 def bench_state(args, lqcd):
     """LatticeStateEFSyncDefault written and read back in the serde bincode
     layout (state.rs:1046-1062): device download + encode + file, and file +
@@ -608,7 +652,7 @@ def main():
     if args.workload != "hmc32":
         run = {"batched": bench_batched, "large": bench_large, "mh3d": bench_mh3d,
                "small": bench_small, "fields": bench_fields,
-               "state": bench_state}[args.workload]
+               "state": bench_state, "stats": bench_stats}[args.workload]
         print(json.dumps(run(args, lqcd)))
         return
 

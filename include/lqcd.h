@@ -401,8 +401,48 @@ typedef enum {
     LQCD_KERNEL_METROPOLIS = 6,
     LQCD_KERNEL_MOMENTA = 7,
     LQCD_KERNEL_FORCE = 8,
-    LQCD_KERNEL_FIELDS = 9
+    LQCD_KERNEL_FIELDS = 9,
+    LQCD_KERNEL_STATS = 10
 } lqcd_kernel_kind;
+
+/* ---------------------------------------------------------- statistics */
+/* The statistics of src/statistics/mod.rs over values resident in device
+ * memory (a device pointer from hipMalloc, or lqcd_field_strength_device's
+ * output): n doubles, or, when complex_values != 0, n complex numbers stored
+ * as (re, im) pairs, in which case every result is a (re, im) pair too.
+ * Complex arithmetic is num-complex's, with no conjugation (the complex
+ * variance is the mean of (x - m)(x - m), as in the reference).  The sums
+ * run on the context's stream in a fixed order: results are the same run to
+ * run, and agree with the reference's sequential sums to rounding.  The calls
+ * synchronize the context's stream. */
+typedef enum {
+    /* the sequential functions: two-pass variance sum (x - mean)^2 / (len - 1)
+     * (mean_and_variance, mod.rs:375-399), covariance
+     * mean(x y) - mean(x) mean(y) (mod.rs:427-458) */
+    LQCD_STATS_SEQUENTIAL = 0,
+    /* the *_par_iter functions: one-pass (sum x^2 - (sum x)^2 / len) / (len - 1)
+     * (mod.rs:163-181), covariance (sum x y - sum x sum y / len) / len
+     * (mod.rs:279-306) */
+    LQCD_STATS_PARALLEL = 1
+} lqcd_stats_form;
+
+/* statistics::mean / mean_par_iter_val (mod.rs:321-335, 57-64): sum / len;
+ * NaN for n = 0.  mean: 1 double, 2 when complex. */
+int lqcd_stats_mean(lqcd_ctx* ctx, const double* d_values, size_t n, int complex_values, double* mean);
+/* mean_and_variance (mod.rs:375-399) or mean_and_variance_par_iter_val
+ * (mod.rs:163-181) by `form`: out = mean, variance (2 doubles, 4 when
+ * complex).  n = 0 is LQCD_ERR_INVALID_ARGUMENT (the reference's len - 1
+ * underflows); n = 1 gives a NaN variance, as in the reference. */
+int lqcd_stats_mean_and_variance(lqcd_ctx* ctx, const double* d_values, size_t n, int complex_values, int form,
+                                 double* out);
+/* mean_with_error / mean_with_error_par_iter_val (mod.rs:401-405, 201-207):
+ * out = mean, sqrt(variance / len), real values only. */
+int lqcd_stats_mean_with_error(lqcd_ctx* ctx, const double* d_values, size_t n, int form, double out[2]);
+/* covariance (mod.rs:427-458) or covariance_par_iter_val (mod.rs:279-306) by
+ * `form`.  *has_value = 0 when nx != ny (the reference's None; out is left
+ * alone), else 1 and out = the covariance (1 double, 2 when complex). */
+int lqcd_stats_covariance(lqcd_ctx* ctx, const double* d_x, size_t nx, const double* d_y, size_t ny,
+                          int complex_values, int form, double* out, int* has_value);
 
 /* Copies `bytes` (a multiple of 16) between two device buffers `iters` times
  * with a streaming kernel and returns the read+write rate in GB/s: the

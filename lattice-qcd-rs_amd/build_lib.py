@@ -18,7 +18,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LQCD_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["capi.cpp", "capi_batch.cpp", "capi_state.cpp", "k_fields.hip", "k_observables.hip", "k_md.hip", "k_mc.hip",
-           "k_batch.hip", "k_gauss.hip", "k_clover.hip"]
+           "k_batch.hip", "k_gauss.hip", "k_clover.hip", "capi_stats.cpp", "k_stats.hip"]
 
 FLAGS = [
     "-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",

@@ -271,6 +271,7 @@ int lqcd_ctx_destroy(lqcd_ctx* ctx)
         (void)hipEventDestroy(e.start);
         (void)hipEventDestroy(e.stop);
     }
+    if (ctx->stats_scratch) (void)hipFree(ctx->stats_scratch);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return LQCD_OK;

@@ -23,6 +23,7 @@ struct lqcd_ctx {
     std::vector<lqcd_prof_event> events;     // recorded launches of the profiled kinds
     std::vector<lqcd_prof_event> event_pool; // reusable event pairs
     unsigned profile_mask = 0;
+    double* stats_scratch = nullptr;        // statistics partials and sums (device)
 };
 
 struct lqcd_lattice {
@@ -83,7 +84,8 @@ enum ProfileKind {
     PROF_MOMENTA = 7,
     PROF_FORCE_STORE = 8,
     PROF_FIELDS = 9,
-    PROF_NKINDS = 10
+    PROF_STATS = 10,
+    PROF_NKINDS = 11
 };
 
 // ---- fields (k_fields.hip) ----
@@ -155,6 +157,17 @@ hipError_t launch_efield_from_double(const lqcd_lattice* L, const double* Ed, vo
 hipError_t launch_final_reduce1(const lqcd_lattice* L, int nblocks, double* d_out1);
 // max over links of max(|U U^+ - 1|_ij, |det U - 1|) of buffer U into d_out1
 hipError_t launch_su3_distance(const lqcd_lattice* L, const void* U, double* d_out1);
+
+// ---- statistics (k_stats.hip) ----
+// sums over n values (complex: (re, im) pairs) of x and x x (STATS_MOMENTS),
+// of (x - m)(x - m) with m = (center[0] + i center[1]) / n (STATS_CENTERED),
+// or of x, y and x y (STATS_COV), written to d_out in that order, one double
+// (real) or a (re, im) pair (complex) each.  partials holds nblocks * 6
+// doubles.
+enum { STATS_MOMENTS = 0, STATS_CENTERED = 1, STATS_COV = 2 };
+int stats_blocks(int64_t n);
+hipError_t launch_stats(lqcd_ctx* ctx, int op, bool cplx, const double* x, const double* y, int64_t n,
+                        const double* center, double* partials, int nblocks, double* d_out);
 
 // profiling hooks (capi.cpp)
 void prof_begin(lqcd_ctx* ctx, int kind, lqcd_prof_event* ev);

@@ -879,3 +879,6 @@ class HybridMethodVec:
         for m in self._methods:
             state = state.monte_carlo_step(m)
         return state
+
+
+from . import statistics  # noqa: E402  (statistics::*, needs Context above)

@@ -42,6 +42,8 @@ KERNEL_METROPOLIS = 6
 KERNEL_MOMENTA = 7
 KERNEL_FORCE = 8
 KERNEL_FIELDS = 9
+KERNEL_STATS = 10
+STATS_SEQUENTIAL, STATS_PARALLEL = 0, 1
 FIELD_CLOVER, FIELD_F_MU_NU, FIELD_MAGNETIC = 0, 1, 2
 
 
@@ -134,6 +136,10 @@ SIGNATURES = {
     "lqcd_f_mu_nu_field": (_I, [_P, _P, _S]),
     "lqcd_magnetic_field_vec": (_I, [_P, _P, _S]),
     "lqcd_field_strength_device": (_I, [_P, _I, _P, _S]),
+    "lqcd_stats_mean": (_I, [_P, _P, _S, _I, _P]),
+    "lqcd_stats_mean_and_variance": (_I, [_P, _P, _S, _I, _I, _P]),
+    "lqcd_stats_mean_with_error": (_I, [_P, _P, _S, _I, _P]),
+    "lqcd_stats_covariance": (_I, [_P, _P, _S, _P, _S, _I, _I, _P, C.POINTER(C.c_int)]),
     "lqcd_md_reconstruct_set": (_I, [_P, _I]),
     "lqcd_batch_reconstruct_set": (_I, [_P, _I]),
     "lqcd_batch_reconstruct_get": (_I, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),

@@ -158,6 +158,11 @@ def _declare(L):
         "or_gauss": (None, [LP, _P, _P, _S, _P]),
         "or_gauss_sum_div": (_D, [LP, _P, _P]),
         "or_project_to_gauss": (_I, [LP, _P, _P, _P, _S, C.POINTER(C.c_size_t)]),
+        "or_stats_mean": (None, [_P, _S, _I, _P]),
+        "or_stats_mean_and_variance": (_I, [_P, _S, _I, _P]),
+        "or_stats_mean_and_variance_par": (_I, [_P, _S, _I, _P]),
+        "or_stats_covariance": (_I, [_P, _S, _P, _S, _I, _P]),
+        "or_stats_covariance_par": (_I, [_P, _S, _P, _S, _I, _P]),
         "or_set_threads": (None, [_I]),
         "or_get_threads": (_I, []),
     }
@@ -577,3 +582,53 @@ def set_threads(n: int) -> None:
 
 def get_threads() -> int:
     return int(lib().or_get_threads())
+
+
+# --------------------------------------------------------------------------
+# statistics (src/statistics/mod.rs): float64 or complex128 1-D arrays
+# --------------------------------------------------------------------------
+def _stats_in(x):
+    x = np.asarray(x)
+    cplx = np.iscomplexobj(x)
+    flat = np.ascontiguousarray(x, dtype=np.complex128 if cplx else np.float64).reshape(-1)
+    return flat, int(cplx)
+
+
+def _stats_out(v: np.ndarray, cplx: int):
+    return complex(v[0], v[1]) if cplx else float(v[0])
+
+
+def stats_mean(x):
+    """statistics::mean (mod.rs:321-335)."""
+    a, c = _stats_in(x)
+    out = np.zeros(2)
+    lib().or_stats_mean(_ptr(a), a.size, c, _ptr(out))
+    return _stats_out(out, c)
+
+
+def stats_mean_and_variance(x, parallel: bool = False):
+    """[mean, variance]: two-pass mean_and_variance (mod.rs:375-399) or the
+    one-pass mean_and_variance_par_iter_val (mod.rs:163-181)."""
+    a, c = _stats_in(x)
+    out = np.zeros(4)
+    f = lib().or_stats_mean_and_variance_par if parallel else lib().or_stats_mean_and_variance
+    if f(_ptr(a), a.size, c, _ptr(out)) != 0:
+        raise ValueError("variance of an empty sequence")
+    k = 2 if c else 1
+    return [_stats_out(out[:k], c), _stats_out(out[k:], c)]
+
+
+def stats_covariance(x, y, parallel: bool = False):
+    """covariance (mod.rs:427-458) or covariance_par_iter_val (mod.rs:279-306);
+    None when the lengths differ."""
+    a, c = _stats_in(x)
+    b, c2 = _stats_in(y)
+    if c != c2:
+        a = a.astype(np.complex128)
+        b = b.astype(np.complex128)
+        c = 1
+    out = np.zeros(2)
+    f = lib().or_stats_covariance_par if parallel else lib().or_stats_covariance
+    if not f(_ptr(a), a.size, _ptr(b), b.size, c, _ptr(out)):
+        return None
+    return _stats_out(out, c)

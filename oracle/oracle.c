@@ -31,6 +31,9 @@
  *                     checkerboard order independence
  *   Bernoulli rule    rand 0.8.5 integer threshold, known values
  *   Gauss projection  gauss_sum_div below threshold after projection
+ *   statistics        the reference's own known answers (statistics/mod.rs
+ *                     doctests and mean_var test), numpy mean / var(ddof=1) /
+ *                     population covariance, one-pass against two-pass forms
  * Parity unpinned (no reference execution possible): the rounding-level
  * behaviour of nalgebra/num-complex internals, and every random stream
  * (StdRng/ChaCha12 and rand_distr's ziggurat are replaced by xoshiro256++ and
@@ -1792,4 +1795,107 @@ int or_project_to_gauss(const or_lattice* l, const double* links, const double* 
     if (iterations) *iterations = it;
     free(tmp);
     return rc;
+}
+
+/* ---- statistics (src/statistics/mod.rs) ------------------------------------
+ * values: n reals, or n complex numbers as (re, im) pairs when cplx != 0.
+ * Complex arithmetic is num-complex's: (a+bi)(c+di) = (ac-bd) + (ad+bc)i, no
+ * conjugation anywhere (the complex "variance" is the mean of (x-m)^2).
+ * Sums run left to right from zero, as Iterator::sum does. */
+
+static or_cplx st_at(const double* x, size_t i, int cplx)
+{
+    or_cplx v = {cplx ? x[2 * i] : x[i], cplx ? x[2 * i + 1] : 0.0};
+    return v;
+}
+static or_cplx st_add(or_cplx a, or_cplx b) { or_cplx r = {a.re + b.re, a.im + b.im}; return r; }
+static or_cplx st_sub(or_cplx a, or_cplx b) { or_cplx r = {a.re - b.re, a.im - b.im}; return r; }
+static or_cplx st_mul(or_cplx a, or_cplx b)
+{
+    or_cplx r = {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+    return r;
+}
+static or_cplx st_div(or_cplx a, double d) { or_cplx r = {a.re / d, a.im / d}; return r; }
+static void st_out(or_cplx v, int cplx, double* out)
+{
+    out[0] = v.re;
+    if (cplx) out[1] = v.im;
+}
+static or_cplx st_sum(const double* x, size_t n, int cplx)
+{
+    or_cplx s = {0.0, 0.0};
+    for (size_t i = 0; i < n; ++i) s = st_add(s, st_at(x, i, cplx));
+    return s;
+}
+
+/* mean, mod.rs:321-335: sum / len (NaN for an empty input) */
+void or_stats_mean(const double* x, size_t n, int cplx, double* out)
+{
+    st_out(st_div(st_sum(x, n, cplx), (double)n), cplx, out);
+}
+
+/* mean_and_variance, mod.rs:375-399: two passes, variance =
+ * sum (x - mean)(x - mean) / (len - 1).  out = mean, variance (1 or 2 doubles
+ * each).  Returns -1 for an empty input ((len - 1) underflows in the
+ * reference). */
+int or_stats_mean_and_variance(const double* x, size_t n, int cplx, double* out)
+{
+    if (n == 0) return -1;
+    const or_cplx m = st_div(st_sum(x, n, cplx), (double)n);
+    or_cplx v = {0.0, 0.0};
+    for (size_t i = 0; i < n; ++i) {
+        const or_cplx d = st_sub(st_at(x, i, cplx), m);
+        v = st_add(v, st_mul(d, d));
+    }
+    st_out(m, cplx, out);
+    st_out(st_div(v, (double)(n - 1)), cplx, out + (cplx ? 2 : 1));
+    return 0;
+}
+
+/* mean_and_variance_par_iter_val, mod.rs:163-181: one pass over (x, x*x),
+ * variance = (sum x^2 - (sum x)(sum x) / len) / (len - 1). */
+int or_stats_mean_and_variance_par(const double* x, size_t n, int cplx, double* out)
+{
+    if (n == 0) return -1;
+    or_cplx s = {0.0, 0.0}, s2 = {0.0, 0.0};
+    for (size_t i = 0; i < n; ++i) {
+        const or_cplx v = st_at(x, i, cplx);
+        s = st_add(s, v);
+        s2 = st_add(s2, st_mul(v, v));
+    }
+    const or_cplx var = st_div(st_sub(s2, st_div(st_mul(s, s), (double)n)), (double)(n - 1));
+    st_out(st_div(s, (double)n), cplx, out);
+    st_out(var, cplx, out + (cplx ? 2 : 1));
+    return 0;
+}
+
+/* covariance, mod.rs:427-458: mean(x y) - mean(x) mean(y).  Returns 1 and
+ * writes out when the lengths agree, 0 (the reference's None) otherwise. */
+int or_stats_covariance(const double* x, size_t nx, const double* y, size_t ny, int cplx, double* out)
+{
+    if (nx != ny) return 0;
+    or_cplx sp = {0.0, 0.0};
+    for (size_t i = 0; i < nx; ++i) sp = st_add(sp, st_mul(st_at(x, i, cplx), st_at(y, i, cplx)));
+    const or_cplx mp = st_div(sp, (double)nx);
+    const or_cplx mx = st_div(st_sum(x, nx, cplx), (double)nx);
+    const or_cplx my = st_div(st_sum(y, ny, cplx), (double)ny);
+    st_out(st_sub(mp, st_mul(mx, my)), cplx, out);
+    return 1;
+}
+
+/* covariance_par_iter_val, mod.rs:279-306: one pass over (x, y, x y),
+ * (sum x y - (sum x)(sum y) / len) / len.  Same return convention. */
+int or_stats_covariance_par(const double* x, size_t nx, const double* y, size_t ny, int cplx, double* out)
+{
+    if (nx != ny) return 0;
+    or_cplx sx = {0.0, 0.0}, sy = {0.0, 0.0}, sp = {0.0, 0.0};
+    for (size_t i = 0; i < nx; ++i) {
+        const or_cplx a = st_at(x, i, cplx), b = st_at(y, i, cplx);
+        sx = st_add(sx, a);
+        sy = st_add(sy, b);
+        sp = st_add(sp, st_mul(a, b));
+    }
+    const double len = (double)nx;
+    st_out(st_div(st_sub(sp, st_div(st_mul(sx, sy), len)), len), cplx, out);
+    return 1;
 }

@@ -169,6 +169,15 @@ double or_gauss_sum_div(const or_lattice* l, const double* links, const double* 
 int    or_project_to_gauss(const or_lattice* l, const double* links, const double* e_in,
                            double* e_out, size_t max_iter, size_t* iterations);
 
+/* ---- statistics (src/statistics/mod.rs) ----
+ * values: n reals, or n complex numbers as (re, im) pairs when cplx != 0 */
+void or_stats_mean(const double* x, size_t n, int cplx, double* out);
+int  or_stats_mean_and_variance(const double* x, size_t n, int cplx, double* out);
+int  or_stats_mean_and_variance_par(const double* x, size_t n, int cplx, double* out);
+int  or_stats_covariance(const double* x, size_t nx, const double* y, size_t ny, int cplx, double* out);
+int  or_stats_covariance_par(const double* x, size_t nx, const double* y, size_t ny, int cplx,
+                             double* out);
+
 /* threads used by the parallel loops (0 = OpenMP default) */
 void or_set_threads(int n);
 int  or_get_threads(void);
