@@ -698,8 +698,8 @@ def main():
     kernels = {}
     for name, kind, reals in [("kick_drift", F.KERNEL_KICK_DRIFT, REALS_KICK_DRIFT),
                               ("kick", F.KERNEL_KICK, REALS_KICK),
-                              ("plaquette", F.KERNEL_PLAQUETTE, REALS_PLAQ),
-                              ("efield_energy", F.KERNEL_EFIELD_ENERGY, REALS_EENERGY),
+                              # H_U and H_E sums in one kernel (plaquette + E energy)
+                              ("hamiltonian_sums", F.KERNEL_PLAQUETTE, REALS_PLAQ + REALS_EENERGY),
                               ("reunitarize", F.KERNEL_REUNIT, REALS_REUNIT),
                               ("momenta", F.KERNEL_MOMENTA, REALS_MOMENTA)]:
         reals32 = {"kick_drift": reals_kd, "kick": REALS_KICK_R12 if r12 else REALS_KICK}.get(name, reals)

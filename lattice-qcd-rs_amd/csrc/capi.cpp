@@ -45,8 +45,10 @@ int read_results(lqcd_lattice* L, int n)
 // plaquette trace sum into host_result[0..1]; E energy into host_result[3].
 int hamiltonian_terms(lqcd_lattice* L, const void* U, bool with_e)
 {
-    HIP_TRY(launch_plaquette_sum(L, U, L->result));
-    if (with_e) HIP_TRY(launch_efield_energy(L, L->E, L->result + 3));
+    if (with_e)
+        HIP_TRY(launch_hamiltonian_sum(L, U, L->E, L->result));
+    else
+        HIP_TRY(launch_plaquette_sum(L, U, L->result));
     return read_results(L, with_e ? 4 : 3);
 }
 
@@ -710,12 +712,15 @@ int lqcd_hmc_trajectory(lqcd_lattice* L, double beta, double dt, size_t n, int d
     CHECK_ARG(drift == LQCD_DRIFT_ADDITIVE || drift == LQCD_DRIFT_EXP, "unknown drift mode");
     CHECK_ARG(scheme == LQCD_SCHEME_REFERENCE || scheme == LQCD_SCHEME_LEAPFROG, "unknown scheme");
     DeviceGuard dg(L->ctx->device);
-    if (int rc = hamiltonian_terms(L, L->U[L->cur], true)) return rc;
-    const double h_old = beta * L->host_result[2] + beta * L->host_result[3];
+    // H_old's sums stay on the device (result[0..3]) until H_new's
+    // (result[4..7]) are done: one host round trip per trajectory
+    HIP_TRY(launch_hamiltonian_sum(L, L->U[L->cur], L->E, L->result));
     int last = L->cur;
     if (int rc = integrate_steps(L, dt, n, drift, scheme, &last)) return rc;
-    if (int rc = hamiltonian_terms(L, L->U[last], true)) return rc;
-    const double h_new = beta * L->host_result[2] + beta * L->host_result[3];
+    HIP_TRY(launch_hamiltonian_sum(L, L->U[last], L->E, L->result + 4));
+    if (int rc = read_results(L, 8)) return rc;
+    const double h_old = beta * L->host_result[2] + beta * L->host_result[3];
+    const double h_new = beta * L->host_result[6] + beta * L->host_result[7];
     // probability_of_replacement, hybrid_monte_carlo.rs:584-589 (clamped twice)
     double p = std::fmax(std::fmin(std::exp(h_old - h_new), 1.0), 0.0);
     p = std::fmax(std::fmin(p, 1.0), 0.0);

@@ -104,6 +104,9 @@ hipError_t launch_stream_copy(hipStream_t stream, const void* src, void* dst, si
 // ---- observables (k_observables.hip) ----
 // out[0..1] = sum Tr P (complex), out[2] = sum (1 - Re Tr P / 3)
 hipError_t launch_plaquette_sum(const lqcd_lattice* L, const void* U, double* d_out3);
+// the plaquette sums (d_out4[0..2], as launch_plaquette_sum) and the E energy
+// sum (d_out4[3]) in one kernel; recorded as a plaquette launch
+hipError_t launch_hamiltonian_sum(const lqcd_lattice* L, const void* U, const void* E, double* d_out4);
 // out[0] = sum_a e_a^2 / 2 over all links
 hipError_t launch_efield_energy(const lqcd_lattice* L, const void* E, double* d_out1);
 // out[0] = sum p, out[1] = number accepted

@@ -22,6 +22,25 @@ __global__ __launch_bounds__(kBlock) void k_plaquette_partials(const vec2_t<S>* 
     block_reduce_to<3>(acc, partials + (int64_t)blockIdx.x * kReduceSlots);
 }
 
+// both terms of H in one pass over the lattice (hamiltonian_total,
+// src/simulation/state.rs:229-231): the plaquette sums of plaquette_site and
+// the E energy sum of the site's D links (acc[3]); the E loads stream while
+// the plaquette's link loads wait
+template <int D, typename S>
+__global__ __launch_bounds__(kBlock) void k_hamiltonian_partials(const vec2_t<S>* __restrict__ U,
+                                                                 const S* __restrict__ E, Geom g,
+                                                                 double* __restrict__ partials)
+{
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const int64_t step = (int64_t)gridDim.x * kBlock;
+    for (int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x; s < g.nsite; s += step) {
+#pragma unroll
+        for (int mu = 0; mu < D; ++mu) acc[3] += efield_energy_link<S>(E, g, mu, s);
+        plaquette_site<D, S>(U, g, s, acc);
+    }
+    block_reduce_to<4>(acc, partials + (int64_t)blockIdx.x * kReduceSlots);
+}
+
 // sum over links of Su3Adjoint::trace_squared = sum_a e_a^2 / 2
 // (src/field.rs:164-167, hamiltonian_efield src/simulation/state.rs:1370-1385)
 template <typename S>
@@ -116,6 +135,18 @@ hipError_t launch_plaquette_sum(const lqcd_lattice* L, const void* U, double* d_
                             (const vec2_t<S>*)U, L->g, L->partials));
     prof_end(L->ctx, &ev);
     k_final_reduce<3><<<1, kBlock, 0, L->ctx->stream>>>(L->partials, nb, d_out3);
+    return hipGetLastError();
+}
+
+hipError_t launch_hamiltonian_sum(const lqcd_lattice* L, const void* U, const void* E, double* d_out4)
+{
+    const int nb = reduce_blocks(L->g.nsite);
+    lqcd_prof_event ev;
+    prof_begin(L->ctx, PROF_PLAQUETTE, &ev);
+    LQCD_DISPATCH_DS(L, k_hamiltonian_partials<D, S><<<nb, kBlock, 0, L->ctx->stream>>>(
+                            (const vec2_t<S>*)U, (const S*)E, L->g, L->partials));
+    prof_end(L->ctx, &ev);
+    k_final_reduce<4><<<1, kBlock, 0, L->ctx->stream>>>(L->partials, nb, d_out4);
     return hipGetLastError();
 }
 
