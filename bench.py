@@ -73,7 +73,7 @@ def parse():
     p.add_argument("--dt", type=float, default=0.025)
     p.add_argument("--beta", type=float, default=6.0)
     p.add_argument("--f64-steps", type=int, default=10, help="trajectories of the FP64 leg")
-    p.add_argument("--cpu-md-steps", type=int, default=3,
+    p.add_argument("--cpu-md-steps", type=int, default=15,
                    help="MD steps of the CPU baseline sample (0 = skip)")
     p.add_argument("--seed", type=int, default=0x457893F44AB067F0)
     p.add_argument("--copy-only", action="store_true",

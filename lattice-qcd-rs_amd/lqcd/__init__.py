@@ -18,6 +18,7 @@ reference (lattice-qcd-rs)              here
 ``OverrelaxationSweepReverse``          ``OverrelaxationSweepReverse`` (checkerboard)
 ``HybridMethodVec``                     ``HybridMethodVec``
 independent chains, one after another   ``BatchedChains`` (one launch per MD step)
+``statistics::*``                       ``lqcd.statistics`` (sums on the device)
 =====================================  ==========================================
 
 Host arrays use the reference's layouts: links ``(nsite*D, 18)`` float64,
