@@ -60,7 +60,8 @@ typedef enum { LQCD_DRIFT_ADDITIVE = 0, LQCD_DRIFT_EXP = 1 } lqcd_drift;
 
 /* REFERENCE = n calls of integrate_symplectic, kick(dt/2) drift(dt)
  * kick(dt/2) each (symplectic_euler_rayon.rs:222-252); LEAPFROG merges the
- * two half kicks between consecutive steps (n + 1 force evaluations). */
+ * two half kicks between consecutive steps (n + 1 force evaluations), which
+ * is simulate_using_leapfrog_n (src/simulation/state.rs:321-358). */
 typedef enum { LQCD_SCHEME_REFERENCE = 0, LQCD_SCHEME_LEAPFROG = 1 } lqcd_scheme;
 
 typedef struct lqcd_ctx lqcd_ctx;
@@ -149,6 +150,27 @@ int lqcd_integrate_link(lqcd_lattice* lat, double dt, int drift);
  * (src/integrator/symplectic_euler_rayon.rs:222-252).  n == 0 ->
  * LQCD_ERR_ZERO_INTEGRATION. */
 int lqcd_simulate_symplectic_n(lqcd_lattice* lat, double dt, size_t n, int drift, int scheme);
+
+/* The other transitions of SymplecticIntegrator (src/integrator/mod.rs:93-208)
+ * as SymplecticEulerRayon implements them.  The handle does not know whether
+ * its E is at a synchronous or a half step (SimulationStateLeap); the caller
+ * keeps track, as the reference's types do.
+ *   sync_sync (symplectic_euler_rayon.rs:127-144): U' = drift(U, E, dt) and
+ *       E' = E + dt F(U), both from the state before the call;
+ *   sync_leap (:173-193): E' = E + dt/2 F(U), U unchanged;
+ *   leap_leap (:146-171): U' = drift(U, E, dt), then E' = E + dt F(U');
+ *   leap_sync (:195-220): U' = drift(U, E, dt), then E' = E + dt/2 F(U').
+ * simulate_sync_n / simulate_leap_n repeat sync_sync / leap_leap n times
+ * (src/simulation/state.rs:397-418, 624-648; n == 0 ->
+ * LQCD_ERR_ZERO_INTEGRATION).  sync_leap, n - 1 leap_leap and leap_sync are
+ * simulate_using_leapfrog_n (state.rs:321-358), which is also
+ * lqcd_simulate_symplectic_n with LQCD_SCHEME_LEAPFROG. */
+int lqcd_integrate_sync_sync(lqcd_lattice* lat, double dt, int drift);
+int lqcd_integrate_sync_leap(lqcd_lattice* lat, double dt);
+int lqcd_integrate_leap_leap(lqcd_lattice* lat, double dt, int drift);
+int lqcd_integrate_leap_sync(lqcd_lattice* lat, double dt, int drift);
+int lqcd_simulate_sync_n(lqcd_lattice* lat, double dt, size_t n, int drift);
+int lqcd_simulate_leap_n(lqcd_lattice* lat, double dt, size_t n, int drift);
 
 /* ------------------------------------------------------------------ HMC */
 typedef struct {

@@ -688,6 +688,119 @@ int lqcd_integrate_link(lqcd_lattice* L, double dt, int drift)
     return LQCD_OK;
 }
 
+// the links after an update of every link by the chosen drift
+static void note_drift(lqcd_lattice* L, int drift)
+{
+    if (drift == LQCD_DRIFT_ADDITIVE)
+        L->su3_known = -2;
+    else if (L->su3_known >= 0)
+        ++L->su3_known;
+}
+
+// n steps of integrate_sync_sync (symplectic_euler_rayon.rs:127-144): both
+// halves from the old state, U' = drift(U, E), E' = E + dt F(U).  The drift
+// writes a second buffer, so the kick still reads the old links.
+static int sync_sync_steps(lqcd_lattice* L, double dt, size_t n, int drift)
+{
+    if (int rc = ensure_md_variant(L, drift)) return rc;
+    for (size_t k = 0; k < n; ++k) {
+        int w1, w2;
+        other_buffers(L->cur, &w1, &w2);
+        HIP_TRY(launch_drift(L, L->U[L->cur], L->U[w1], L->E, dt, drift));
+        HIP_TRY(launch_kick(L, L->U[L->cur], L->E, dt));
+        L->cur = w1;
+        note_drift(L, drift);
+    }
+    return LQCD_OK;
+}
+
+// n steps of integrate_leap_leap (symplectic_euler_rayon.rs:146-171), drift
+// then kick with the new links; the last step's kick is `last_kick` * dt
+// (0.5 for integrate_leap_sync, :195-220).  A kick followed by the next
+// step's drift is one fused launch.
+static int leap_steps(lqcd_lattice* L, double dt, size_t n, int drift, double last_kick)
+{
+    if (int rc = ensure_md_variant(L, drift)) return rc;
+    int w1, w2;
+    other_buffers(L->cur, &w1, &w2);
+    int src = L->cur, dst = w1;
+    HIP_TRY(launch_drift(L, L->U[src], L->U[dst], L->E, dt, drift));
+    for (size_t k = 1; k < n; ++k) {
+        src = dst;
+        dst = (dst == w1) ? w2 : w1;
+        HIP_TRY(launch_kick_drift(L, L->U[src], L->U[dst], L->E, dt, dt, drift));
+    }
+    HIP_TRY(launch_kick(L, L->U[dst], L->E, last_kick * dt));
+    L->cur = dst;
+    for (size_t k = 0; k < n; ++k) note_drift(L, drift);
+    return LQCD_OK;
+}
+
+static int check_integration(lqcd_lattice* L, double dt, int drift)
+{
+    CHECK_ARG(L, "lattice is null");
+    CHECK_ARG(std::isfinite(dt), "dt must be finite");
+    CHECK_ARG(drift == LQCD_DRIFT_ADDITIVE || drift == LQCD_DRIFT_EXP, "unknown drift mode");
+    return LQCD_OK;
+}
+
+int lqcd_integrate_sync_sync(lqcd_lattice* L, double dt, int drift)
+{
+    if (int rc = check_integration(L, dt, drift)) return rc;
+    DeviceGuard dg(L->ctx->device);
+    if (int rc = sync_sync_steps(L, dt, 1, drift)) return rc;
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    return LQCD_OK;
+}
+
+int lqcd_simulate_sync_n(lqcd_lattice* L, double dt, size_t n, int drift)
+{
+    if (int rc = check_integration(L, dt, drift)) return rc;
+    if (n == 0) return fail(LQCD_ERR_ZERO_INTEGRATION, "no integration steps");
+    DeviceGuard dg(L->ctx->device);
+    if (int rc = sync_sync_steps(L, dt, n, drift)) return rc;
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    return LQCD_OK;
+}
+
+int lqcd_integrate_sync_leap(lqcd_lattice* L, double dt)
+{
+    if (int rc = check_integration(L, dt, LQCD_DRIFT_EXP)) return rc;
+    DeviceGuard dg(L->ctx->device);
+    if (int rc = ensure_md_variant(L, LQCD_DRIFT_EXP)) return rc;
+    HIP_TRY(launch_kick(L, L->U[L->cur], L->E, dt / 2.0));
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    return LQCD_OK;
+}
+
+int lqcd_integrate_leap_leap(lqcd_lattice* L, double dt, int drift)
+{
+    if (int rc = check_integration(L, dt, drift)) return rc;
+    DeviceGuard dg(L->ctx->device);
+    if (int rc = leap_steps(L, dt, 1, drift, 1.0)) return rc;
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    return LQCD_OK;
+}
+
+int lqcd_simulate_leap_n(lqcd_lattice* L, double dt, size_t n, int drift)
+{
+    if (int rc = check_integration(L, dt, drift)) return rc;
+    if (n == 0) return fail(LQCD_ERR_ZERO_INTEGRATION, "no integration steps");
+    DeviceGuard dg(L->ctx->device);
+    if (int rc = leap_steps(L, dt, n, drift, 1.0)) return rc;
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    return LQCD_OK;
+}
+
+int lqcd_integrate_leap_sync(lqcd_lattice* L, double dt, int drift)
+{
+    if (int rc = check_integration(L, dt, drift)) return rc;
+    DeviceGuard dg(L->ctx->device);
+    if (int rc = leap_steps(L, dt, 1, drift, 0.5)) return rc;
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    return LQCD_OK;
+}
+
 int lqcd_simulate_symplectic_n(lqcd_lattice* L, double dt, size_t n, int drift, int scheme)
 {
     CHECK_ARG(L, "lattice is null");

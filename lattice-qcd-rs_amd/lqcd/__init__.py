@@ -10,6 +10,7 @@ reference (lattice-qcd-rs)              here
 ``LatticeCyclic::<D>::new(size, dim)``  ``LatticeCyclic(size, dim, d)``
 ``LatticeStateDefault<D>``              ``LatticeStateDefault`` (device resident)
 ``LatticeStateEFSyncDefault``           ``LatticeStateEFSyncDefault``
+``SimulationStateLeap``                 ``SimulationStateLeap`` (E half a step ahead)
 ``SymplecticEulerRayon``                ``SymplecticEulerGpu``
 ``HybridMonteCarloDiagnostic``          ``HybridMonteCarloDiagnostic``
 ``MetropolisHastingsSweep``             ``MetropolisHastingsSweep`` (checkerboard)
@@ -38,7 +39,7 @@ from ._ffi import (DRIFT_ADDITIVE, DRIFT_EXP, F32, F64, SCHEME_LEAPFROG, SCHEME_
 
 __all__ = [
     "Context", "LatticeCyclic", "LatticeStateDefault", "LatticeStateEFSyncDefault",
-    "SymplecticEulerGpu", "HybridMonteCarloDiagnostic", "MetropolisHastingsSweep",
+    "SymplecticEulerGpu", "SimulationStateLeap", "HybridMonteCarloDiagnostic", "MetropolisHastingsSweep",
     "BatchedChains", "ChainSpec", "HeatBathSweep", "OverrelaxationSweepRotation",
     "OverrelaxationSweepReverse", "HybridMethodVec",
     "LatticeInitializationError", "StateInitializationError", "MultiIntegrationError",
@@ -518,8 +519,107 @@ class LatticeStateEFSyncDefault:
         self._t += numbers_of_times
         return self
 
+    def simulate_symplectic_n_auto(self, integrator: "SymplecticEulerGpu", delta_t: float,
+                                   numbers_of_times: int):
+        """``simulate_symplectic_n_auto`` (src/simulation/state.rs:500-511)."""
+        return self.simulate_symplectic_n(integrator, delta_t, numbers_of_times)
+
+    def simulate_sync(self, integrator: "SymplecticEulerGpu", delta_t: float):
+        """``simulate_sync`` (src/simulation/state.rs:383-390): one
+        ``integrate_sync_sync``; advances the state in place."""
+        return integrator.integrate_sync_sync(self, delta_t)
+
+    def simulate_sync_n(self, integrator: "SymplecticEulerGpu", delta_t: float, numbers_of_times: int):
+        """``simulate_sync_n`` (src/simulation/state.rs:397-418)."""
+        if numbers_of_times == 0:
+            raise MultiIntegrationError("no integration steps")
+        _raise(F.lib().lqcd_simulate_sync_n(self.handle, delta_t, numbers_of_times, integrator.drift))
+        self._t += numbers_of_times
+        return self
+
+    def simulate_to_leapfrog(self, integrator: "SymplecticEulerGpu", delta_t: float) -> "SimulationStateLeap":
+        """``simulate_to_leapfrog`` (src/simulation/state.rs:302-313): half a
+        momentum step.  The returned leap state owns the device fields; this
+        state is consumed."""
+        return integrator.integrate_sync_leap(self, delta_t)
+
+    def simulate_using_leapfrog_n(self, integrator: "SymplecticEulerGpu", delta_t: float,
+                                  number_of_steps: int):
+        """``simulate_using_leapfrog_n`` (src/simulation/state.rs:321-358):
+        half step, number_of_steps - 1 leap steps, half step back, as one
+        sequence of launches (``lqcd_simulate_symplectic_n`` with
+        ``SCHEME_LEAPFROG``)."""
+        if number_of_steps == 0:
+            raise MultiIntegrationError("no integration steps")
+        _raise(F.lib().lqcd_simulate_symplectic_n(self.handle, delta_t, number_of_steps,
+                                                  integrator.drift, SCHEME_LEAPFROG))
+        self._t += number_of_steps
+        return self
+
+    simulate_using_leapfrog_n_auto = simulate_using_leapfrog_n
+
     def monte_carlo_step(self, m):
         return m.next_element(self._state)
+
+
+class SimulationStateLeap:
+    """``SimulationStateLeap<State, D>`` (src/simulation/state.rs:856-1040):
+    the state with its E half a step ahead of its links.  It shares the
+    device fields of the synchronous state it was made from."""
+
+    def __init__(self, sync_state: LatticeStateEFSyncDefault):
+        self._sync = sync_state
+
+    @property
+    def handle(self):
+        return self._sync.handle
+
+    def lattice_state(self) -> LatticeStateDefault:
+        return self._sync.lattice_state()
+
+    def lattice(self) -> LatticeCyclic:
+        return self._sync.lattice()
+
+    def beta(self) -> float:
+        return self._sync.beta()
+
+    def t(self) -> int:
+        return self._sync.t()
+
+    def link_matrix(self) -> np.ndarray:
+        return self._sync.link_matrix()
+
+    def e_field(self) -> np.ndarray:
+        return self._sync.e_field()
+
+    def hamiltonian_links(self) -> float:
+        return self._sync.hamiltonian_links()
+
+    def hamiltonian_efield(self) -> float:
+        return self._sync.hamiltonian_efield()
+
+    def hamiltonian_total(self) -> float:
+        return self._sync.hamiltonian_total()
+
+    def simulate_leap(self, integrator: "SymplecticEulerGpu", delta_t: float) -> "SimulationStateLeap":
+        """``simulate_leap`` (src/simulation/state.rs:577-584): one
+        ``integrate_leap_leap``."""
+        return integrator.integrate_leap_leap(self, delta_t)
+
+    def simulate_leap_n(self, integrator: "SymplecticEulerGpu", delta_t: float,
+                        numbers_of_times: int) -> "SimulationStateLeap":
+        """``simulate_leap_n`` (src/simulation/state.rs:624-648)."""
+        if numbers_of_times == 0:
+            raise MultiIntegrationError("no integration steps")
+        _raise(F.lib().lqcd_simulate_leap_n(self.handle, delta_t, numbers_of_times, integrator.drift))
+        self._sync._t += numbers_of_times
+        return self
+
+    def simulate_to_synchronous(self, integrator: "SymplecticEulerGpu",
+                                delta_t: float) -> LatticeStateEFSyncDefault:
+        """``simulate_to_synchronous`` (src/simulation/state.rs:528-539): one
+        ``integrate_leap_sync``."""
+        return integrator.integrate_leap_sync(self, delta_t)
 
 
 class SymplecticEulerGpu:
@@ -547,6 +647,28 @@ class SymplecticEulerGpu:
     def integrate_link(self, state, delta_t: float):
         _raise(F.lib().lqcd_integrate_link(state.handle, delta_t, self.drift))
         return state
+
+    # the four transitions of SymplecticIntegrator (src/integrator/mod.rs:93-208)
+    # as SymplecticEulerRayon implements them (symplectic_euler_rayon.rs:127-220)
+    def integrate_sync_sync(self, state: LatticeStateEFSyncDefault, delta_t: float):
+        _raise(F.lib().lqcd_integrate_sync_sync(state.handle, delta_t, self.drift))
+        state._t += 1
+        return state
+
+    def integrate_sync_leap(self, state: LatticeStateEFSyncDefault, delta_t: float) -> SimulationStateLeap:
+        # the time counter does not advance
+        _raise(F.lib().lqcd_integrate_sync_leap(state.handle, delta_t))
+        return SimulationStateLeap(state)
+
+    def integrate_leap_leap(self, state: SimulationStateLeap, delta_t: float) -> SimulationStateLeap:
+        _raise(F.lib().lqcd_integrate_leap_leap(state.handle, delta_t, self.drift))
+        state._sync._t += 1
+        return state
+
+    def integrate_leap_sync(self, state: SimulationStateLeap, delta_t: float) -> LatticeStateEFSyncDefault:
+        _raise(F.lib().lqcd_integrate_leap_sync(state.handle, delta_t, self.drift))
+        state._sync._t += 1
+        return state._sync
 
 
 def _splitmix64(x: int):

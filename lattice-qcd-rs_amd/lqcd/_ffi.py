@@ -125,6 +125,12 @@ SIGNATURES = {
     "lqcd_integrate_efield": (_I, [_P, _D]),
     "lqcd_integrate_link": (_I, [_P, _D, _I]),
     "lqcd_simulate_symplectic_n": (_I, [_P, _D, _S, _I, _I]),
+    "lqcd_integrate_sync_sync": (_I, [_P, _D, _I]),
+    "lqcd_integrate_sync_leap": (_I, [_P, _D]),
+    "lqcd_integrate_leap_leap": (_I, [_P, _D, _I]),
+    "lqcd_integrate_leap_sync": (_I, [_P, _D, _I]),
+    "lqcd_simulate_sync_n": (_I, [_P, _D, _S, _I]),
+    "lqcd_simulate_leap_n": (_I, [_P, _D, _S, _I]),
     "lqcd_hmc_trajectory": (_I, [_P, _D, _D, _S, _I, _I, _U64, C.POINTER(HmcStats)]),
     "lqcd_wilson_forward": (_I, [_P, _D, C.POINTER(_D)]),
     "lqcd_wilson_backward": (_I, [_P, _D, _P, _S]),

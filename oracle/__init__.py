@@ -372,6 +372,26 @@ class OracleLattice:
         lib().or_integrate_link(self.ref, _ptr(u), _ptr(e), dt, mode, _ptr(out))
         return out
 
+    # SymplecticEulerRayon's transitions (src/integrator/symplectic_euler_rayon.rs),
+    # each composed from integrate_link / integrate_efield in the reference's order
+    def integrate_sync_sync(self, u, e, dt, mode=DRIFT_ADDITIVE):
+        """:127-144: links and momenta both from the state before the step."""
+        return self.integrate_link(u, e, dt, mode), self.integrate_efield(u, e, dt)
+
+    def integrate_sync_leap(self, u, e, dt):
+        """:173-193: half a momentum step, links unchanged."""
+        return np.array(u, copy=True), self.integrate_efield(u, e, dt / 2.0)
+
+    def integrate_leap_leap(self, u, e, dt, mode=DRIFT_ADDITIVE):
+        """:146-171: links first, then momenta from the new links."""
+        u2 = self.integrate_link(u, e, dt, mode)
+        return u2, self.integrate_efield(u2, e, dt)
+
+    def integrate_leap_sync(self, u, e, dt, mode=DRIFT_ADDITIVE):
+        """:195-220: links, then half a momentum step from the new links."""
+        u2 = self.integrate_link(u, e, dt, mode)
+        return u2, self.integrate_efield(u2, e, dt / 2.0)
+
     def simulate_symplectic_n(self, u, e, dt, n, mode=DRIFT_ADDITIVE, scheme=SCHEME_REFERENCE):
         u = np.array(u, copy=True)
         e = np.array(e, copy=True)

@@ -263,3 +263,35 @@ def test_parallel_first_call_matches_serial(tmp_path):
     finally:
         O.set_threads(0)
     assert np.array_equal(par, np.concatenate([u2.ravel(), e2.ravel()]))
+
+
+@pytest.mark.parametrize("mode", [O.DRIFT_ADDITIVE, O.DRIFT_EXP])
+def test_leap_transitions_chain_is_the_leapfrog_scheme(mode):
+    # simulate_using_leapfrog_n (state.rs:321-358) = sync_leap, n - 1 leap_leap,
+    # leap_sync; the C loop of the leapfrog scheme is written independently
+    ol = O.OracleLattice(3, [4, 4, 6])
+    u = ol.hot_start(3)
+    e = ol.random_efield(4, 0.3)
+    a, b = ol.integrate_sync_leap(u, e, 0.02)
+    for _ in range(4):
+        a, b = ol.integrate_leap_leap(a, b, 0.02, mode)
+    a, b = ol.integrate_leap_sync(a, b, 0.02, mode)
+    u_ref, e_ref = ol.simulate_symplectic_n(u, e, 0.02, 5, mode, O.SCHEME_LEAPFROG)
+    assert np.array_equal(a, u_ref) and np.array_equal(b, e_ref)
+
+
+def test_sync_sync_is_first_order():
+    # integrate_sync_sync (symplectic_euler_rayon.rs:127-144) is an explicit
+    # Euler step: its energy error over a fixed time falls like dt, the
+    # symplectic step's like dt^2
+    ol = O.OracleLattice(3, [4, 4, 4])
+    u0 = ol.hot_start(5)
+    e0 = ol.random_efield(6, 0.3)
+    errs = []
+    for dt, n in ((0.02, 5), (0.01, 10)):
+        u, e = u0, e0
+        h0 = ol.hamiltonian_links(u, 6.0) + ol.hamiltonian_efield(e, 6.0)
+        for _ in range(n):
+            u, e = ol.integrate_sync_sync(u, e, dt, O.DRIFT_EXP)
+        errs.append(abs(ol.hamiltonian_links(u, 6.0) + ol.hamiltonian_efield(e, 6.0) - h0))
+    assert 1.6 < errs[0] / errs[1] < 2.6, errs
