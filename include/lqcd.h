@@ -226,6 +226,18 @@ int lqcd_metropolis_sweep_random(lqcd_lattice* lat, double beta, size_t n_update
                                  double spread, uint64_t seed, uint64_t counter,
                                  lqcd_mh_stats* stats);
 
+/* One MetropolisHastingsDeltaDiagnostic::next_element
+ * (src/simulation/monte_carlo/metropolis_hastings.rs:374-419): a link drawn
+ * uniformly, its proposal orthonormalize(random_su3_close_to_unity(spread))
+ * U_old, dS from the staple (monte_carlo/mod.rs:324-362), accepted with
+ * p = clamp(exp(-dS), 0, 1) by the rand 0.8 Bernoulli rule.  The draws come
+ * from the device Philox stream (seed, counter = the step): the link from
+ * block 62 of link 0, the proposal and accept word of that link as in
+ * lqcd_metropolis_sweep_random.  Outputs (each may be NULL): the probability
+ * (prob_replace_last), whether it was accepted (has_replace_last) and the
+ * link index site * D + mu. */
+int lqcd_metropolis_link_random(lqcd_lattice* lat, double beta, double spread, uint64_t seed, uint64_t counter,
+                                double* probability, int* accepted, size_t* link);
 /* HeatBathSweep::next_element (src/simulation/monte_carlo/heat_bath.rs:73-123)
  * in checkerboard order (as lqcd_metropolis_sweep), double arithmetic: per
  * link, three SU(2) subgroup heat baths on the r, s, t sub-blocks of U A

@@ -936,6 +936,24 @@ int lqcd_metropolis_sweep(lqcd_lattice* L, double beta, size_t n_update, const d
     return metropolis_finish(L, stats);
 }
 
+int lqcd_metropolis_link_random(lqcd_lattice* L, double beta, double spread, uint64_t seed, uint64_t counter,
+                                double* probability, int* accepted, size_t* link)
+{
+    CHECK_ARG(L, "lattice is null");
+    CHECK_ARG(std::isfinite(beta), "beta must be finite");
+    // MetropolisHastingsDeltaDiagnostic::new returns None outside (0, 1)
+    CHECK_ARG(spread > 0.0 && spread < 1.0, "spread must be in (0, 1)");
+    DeviceGuard dg(L->ctx->device);
+    HIP_TRY(launch_metropolis_link(L, L->U[L->cur], beta, spread, seed, counter, L->result));
+    if (int rc = read_results(L, 3)) return rc;
+    if (probability) *probability = L->host_result[0];
+    if (accepted) *accepted = L->host_result[1] != 0.0 ? 1 : 0;
+    if (link) *link = (size_t)L->host_result[2];
+    // the proposal is orthonormalised: one SU(3)-preserving update
+    if (L->host_result[1] != 0.0 && L->su3_known >= 0) ++L->su3_known;
+    return LQCD_OK;
+}
+
 int lqcd_heat_bath_sweep(lqcd_lattice* L, double beta, uint64_t seed, uint64_t counter, size_t* exhausted)
 {
     CHECK_ARG(L, "lattice is null");

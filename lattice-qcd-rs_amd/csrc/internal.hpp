@@ -143,6 +143,10 @@ hipError_t launch_heat_bath_class(const lqcd_lattice* L, void* U, int mu, int pa
                                   uint64_t counter);
 // (reverse: OverrelaxationSweepReverse, else OverrelaxationSweepRotation)
 hipError_t launch_overrelax_class(const lqcd_lattice* L, void* U, int mu, int parity, bool reverse);
+// one MetropolisHastingsDeltaDiagnostic step on the device stream (seed,
+// counter); d_out3 = probability, accepted, link index
+hipError_t launch_metropolis_link(const lqcd_lattice* L, void* U, double beta, double spread, uint64_t seed,
+                                  uint64_t counter, double* d_out3);
 
 // clover (mode 0), f_mu_nu (1) or magnetic_field_vec (2) of U[cur] at every
 // site, written to d_out in the host layout (k_clover.hip)

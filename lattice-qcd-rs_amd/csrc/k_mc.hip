@@ -181,6 +181,43 @@ __global__ __launch_bounds__(kBlock) void k_metropolis(vec2_t<S>* __restrict__ U
     if (accept) store_link<double, S>(U, g.npad, mu, s, nw);
 }
 
+// MetropolisHastingsDeltaDiagnostic::next_element_default
+// (metropolis_hastings.rs:394-419, potential_modif :374-391): one link drawn
+// uniformly, new link = orthonormalize(random_su3_close_to_unity(spread)) U,
+// dS of delta_s_old_new_cmp (monte_carlo/mod.rs:324-334), accepted with
+// p = clamp(exp(-dS), 0, 1).  Draws of step `counter`: the link from Philox
+// block 62 of link 0 (site = (x y) * nsite >> 64 over the 64-bit word of
+// x, y; direction = z * D >> 32), then the proposal and accept word of that
+// link as in the device sweep.  One lane works; out = p, accepted, link.
+template <int D, typename S>
+__global__ __launch_bounds__(64) void k_metropolis_link(vec2_t<S>* __restrict__ U, Geom g, double beta,
+                                                        double spread, uint64_t seed, uint64_t counter,
+                                                        double* __restrict__ out)
+{
+    if (threadIdx.x != 0) return;
+    const uint4 o = philox_link(0, 62u, counter, seed);
+    const uint64_t w = ((uint64_t)o.x << 32) | (uint64_t)o.y;
+    const int64_t s = (int64_t)__umul64hi(w, (uint64_t)g.nsite);
+    const int mu = (int)(((uint64_t)o.z * (uint64_t)D) >> 32);
+    mat3<double> a, r, nw;
+    mc_staple<D, S>(U, g, mu, s, a);
+    const mat3<double> u = load_link<double, S>(U, g.npad, mu, s);
+    const int64_t li = s * D + mu;
+    device_proposal((uint64_t)li, 0, seed, counter, spread, r);
+    mul<false, false>(r, u, nw);
+    mat3<double> diff;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) diff.e[e] = {nw.e[e].re - u.e[e].re, nw.e[e].im - u.e[e].im};
+    const double ds = -trace_mul<false, false>(diff, a).re * beta / 3.0;
+    const double p = fmax(fmin(exp(-ds), 1.0), 0.0);
+    bool accept = true;
+    if (p < 1.0) accept = device_accept_word((uint64_t)li, seed, counter) < (uint64_t)(p * 18446744073709551616.0);
+    out[0] = p;
+    out[1] = accept ? 1.0 : 0.0;
+    out[2] = (double)li;
+    if (accept) store_link<double, S>(U, g.npad, mu, s, nw);
+}
+
 // ---------------------------------------------------------------- heat bath
 // HeatBathSweep (simulation/monte_carlo/heat_bath.rs:73-123) with draws from
 // the device stream: subgroup g (0 r, 1 s, 2 t) of link l in sweep n uses
@@ -526,6 +563,14 @@ hipError_t launch_metropolis_class(const lqcd_lattice* L, void* U, int mu, int p
                             (vec2_t<S>*)U, L->g, mu, parity, beta, (int)n_update, d_proposals,
                             d_uniforms, 0.0, 0, 0, L->mh_p, L->mh_acc));
     prof_end(L->ctx, &ev);
+    return hipGetLastError();
+}
+
+hipError_t launch_metropolis_link(const lqcd_lattice* L, void* U, double beta, double spread, uint64_t seed,
+                                  uint64_t counter, double* d_out3)
+{
+    LQCD_DISPATCH_DS(L, k_metropolis_link<D, S><<<1, 64, 0, L->ctx->stream>>>(
+                            (vec2_t<S>*)U, L->g, beta, spread, seed, counter, d_out3));
     return hipGetLastError();
 }
 

@@ -14,10 +14,13 @@ reference (lattice-qcd-rs)              here
 ``SymplecticEulerRayon``                ``SymplecticEulerGpu``
 ``HybridMonteCarloDiagnostic``          ``HybridMonteCarloDiagnostic``
 ``MetropolisHastingsSweep``             ``MetropolisHastingsSweep`` (checkerboard)
+``MetropolisHastingsDeltaDiagnostic``   ``MetropolisHastingsDeltaDiagnostic`` (device stream)
 ``HeatBathSweep``                       ``HeatBathSweep`` (checkerboard, device stream)
 ``OverrelaxationSweepRotation``         ``OverrelaxationSweepRotation`` (checkerboard)
 ``OverrelaxationSweepReverse``          ``OverrelaxationSweepReverse`` (checkerboard)
 ``HybridMethodVec``                     ``HybridMethodVec``
+``HybridMethodCouple``                  ``HybridMethodCouple``
+``HybridMonteCarlo``                    ``HybridMonteCarlo``
 independent chains, one after another   ``BatchedChains`` (one launch per MD step)
 ``statistics::*``                       ``lqcd.statistics`` (sums on the device)
 =====================================  ==========================================
@@ -41,7 +44,8 @@ __all__ = [
     "Context", "LatticeCyclic", "LatticeStateDefault", "LatticeStateEFSyncDefault",
     "SymplecticEulerGpu", "SimulationStateLeap", "HybridMonteCarloDiagnostic", "MetropolisHastingsSweep",
     "BatchedChains", "ChainSpec", "HeatBathSweep", "OverrelaxationSweepRotation",
-    "OverrelaxationSweepReverse", "HybridMethodVec",
+    "OverrelaxationSweepReverse", "HybridMethodVec", "HybridMethodCouple", "HybridMonteCarlo",
+    "MetropolisHastingsDeltaDiagnostic",
     "LatticeInitializationError", "StateInitializationError", "MultiIntegrationError",
     "LqcdError", "F32", "F64", "DRIFT_ADDITIVE", "DRIFT_EXP", "SCHEME_REFERENCE",
     "SCHEME_LEAPFROG",
@@ -745,6 +749,12 @@ class HybridMonteCarloDiagnostic:
         return state
 
 
+class HybridMonteCarlo(HybridMonteCarloDiagnostic):
+    """``HybridMonteCarlo`` (src/simulation/monte_carlo/hybrid_monte_carlo.rs:64-200):
+    the trajectory of ``HybridMonteCarloDiagnostic``; the reference's plain
+    type only lacks the two diagnostic getters."""
+
+
 class MetropolisHastingsSweep:
     """``MetropolisHastingsSweep`` (src/simulation/monte_carlo/metropolis_hastings_sweep.rs),
     checkerboard order on the device.  ``sweep`` takes host-given proposals and
@@ -937,6 +947,48 @@ class BatchedChains:
         return bool(used.value)
 
 
+class MetropolisHastingsDeltaDiagnostic:
+    """``MetropolisHastingsDeltaDiagnostic``
+    (src/simulation/monte_carlo/metropolis_hastings.rs:296-427): one link
+    drawn uniformly per step, proposal orthonormalize(random_su3_close_to_unity
+    (spread)) U, accepted with p = min(1, exp(-dS)).  The draws come from the
+    device stream keyed by ``seed`` (the reference's ``rng``), one counter per
+    step."""
+
+    def __init__(self, spread: float, seed: int = 0):
+        # ::new returns None outside (0, 1) (:341-351)
+        if not (0.0 < spread < 1.0):
+            raise ValueError("spread must be in (0, 1)")
+        self._spread = float(spread)
+        self._seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+        self._counter = 0
+        self._prob_replace_last = 0.0
+        self._has_replace_last = False
+        self.last_link: Optional[int] = None
+
+    def spread(self) -> float:
+        return self._spread
+
+    def prob_replace_last(self) -> float:
+        return self._prob_replace_last
+
+    def has_replace_last(self) -> bool:
+        return self._has_replace_last
+
+    def counter(self) -> int:
+        return self._counter
+
+    def next_element(self, state: LatticeStateDefault) -> LatticeStateDefault:
+        p, acc, link = C.c_double(), C.c_int(), C.c_size_t()
+        _raise(F.lib().lqcd_metropolis_link_random(state.handle, state.beta(), self._spread, self._seed,
+                                                   self._counter, C.byref(p), C.byref(acc), C.byref(link)))
+        self._counter += 1
+        self._prob_replace_last = float(p.value)
+        self._has_replace_last = bool(acc.value)
+        self.last_link = int(link.value)
+        return state
+
+
 class HeatBathSweep:
     """``HeatBathSweep`` (src/simulation/monte_carlo/heat_bath.rs:20-123) on the
     device, checkerboard order.  The reference's ``rng`` is a Philox stream
@@ -981,6 +1033,28 @@ class OverrelaxationSweepReverse:
     def next_element(self, state: LatticeStateDefault) -> LatticeStateDefault:
         _raise(F.lib().lqcd_overrelaxation_reverse_sweep(state.handle))
         return state
+
+
+class HybridMethodCouple:
+    """``HybridMethodCouple`` (src/simulation/monte_carlo/hybrid.rs:327-420):
+    ``method_1`` then ``method_2``, each through ``monte_carlo_step``."""
+
+    def __init__(self, method_1, method_2):
+        self._method_1 = method_1
+        self._method_2 = method_2
+
+    def method_1(self):
+        return self._method_1
+
+    def method_2(self):
+        return self._method_2
+
+    def deconstruct(self):
+        return self._method_1, self._method_2
+
+    def next_element(self, state):
+        state = state.monte_carlo_step(self._method_1)
+        return state.monte_carlo_step(self._method_2)
 
 
 class HybridMethodVec:

@@ -137,6 +137,8 @@ SIGNATURES = {
     "lqcd_metropolis_sweep": (_I, [_P, _D, _S, _P, _P, C.POINTER(MhStats)]),
     "lqcd_metropolis_sweep_random": (_I, [_P, _D, _S, _D, _U64, _U64, C.POINTER(MhStats)]),
     "lqcd_heat_bath_sweep": (_I, [_P, _D, _U64, _U64, C.POINTER(C.c_size_t)]),
+    "lqcd_metropolis_link_random": (_I, [_P, _D, _D, _U64, _U64, C.POINTER(_D), C.POINTER(C.c_int),
+                                         C.POINTER(C.c_size_t)]),
     "lqcd_overrelaxation_sweep": (_I, [_P]),
     "lqcd_clover_field": (_I, [_P, _P, _S]),
     "lqcd_f_mu_nu_field": (_I, [_P, _P, _S]),

@@ -474,3 +474,59 @@ def test_overrelaxation_reverse_sweep(dim, ext):
     with pytest.raises(lqcd.LqcdError):
         _, _, odd = make_state(3, [3, 4, 4])
         odd.monte_carlo_step(lqcd.OverrelaxationSweepReverse())
+
+
+def test_hybrid_couple_and_plain_hmc():
+    # HybridMethodCouple (hybrid.rs:327-420) applies its two methods in order:
+    # the same as HybridMethodVec of both, and as the two steps by hand;
+    # HybridMonteCarlo (hybrid_monte_carlo.rs:64-200) moves the state as
+    # HybridMonteCarloDiagnostic with the same seed
+    announce("HybridMethodCouple, HybridMonteCarlo")
+    states = [make_state(4, [4, 4, 4, 4], 5.5)[2] for _ in range(3)]
+    states[0].monte_carlo_step(lqcd.HybridMethodCouple(lqcd.HeatBathSweep(9), lqcd.OverrelaxationSweepRotation()))
+    states[1].monte_carlo_step(lqcd.HybridMethodVec([lqcd.HeatBathSweep(9), lqcd.OverrelaxationSweepRotation()]))
+    states[2].monte_carlo_step(lqcd.HeatBathSweep(9)).monte_carlo_step(lqcd.OverrelaxationSweepRotation())
+    assert np.array_equal(states[0].link_matrix(), states[1].link_matrix())
+    assert np.array_equal(states[0].link_matrix(), states[2].link_matrix())
+    a, b = (make_state(4, [4, 4, 4, 4], 6.0)[2] for _ in range(2))
+    integ = lqcd.SymplecticEulerGpu(lqcd.DRIFT_EXP, lqcd.SCHEME_LEAPFROG)
+    hmc_a = lqcd.HybridMonteCarlo(0.02, 10, integ, 13)
+    hmc_b = lqcd.HybridMonteCarloDiagnostic(0.02, 10, integ, 13)
+    for _ in range(3):
+        a.monte_carlo_step(hmc_a)
+        b.monte_carlo_step(hmc_b)
+    assert np.array_equal(a.link_matrix(), b.link_matrix())
+
+
+@pytest.mark.parametrize("dim,ext", [(4, [4, 4, 4, 6]), (3, [6, 4, 4])])
+def test_metropolis_delta_diagnostic(dim, ext):
+    # MetropolisHastingsDeltaDiagnostic (metropolis_hastings.rs:374-419), one
+    # link per step from the device stream: the oracle restates the link draw
+    # (Philox block 62 of link 0) and takes the proposal and accept word of
+    # that link from its restatement of the sweep's stream; dS from its staple
+    announce(f"MetropolisHastingsDeltaDiagnostic D={dim} ext={ext}")
+    beta, seed, spread = 5.0, 0x1234_5678_9ABC, 0.3
+    ol, u, st = make_state(dim, ext, beta)
+    mh = lqcd.MetropolisHastingsDeltaDiagnostic(spread, seed)
+    accepted = 0
+    for step in range(40):
+        st.monte_carlo_step(mh)
+        o = O.philox4x32([0, 62 << 16, step & 0xFFFFFFFF, step >> 32], [seed & 0xFFFFFFFF, seed >> 32])
+        site = (((int(o[0]) << 32) | int(o[1])) * ol.nsite) >> 64
+        mu = (int(o[2]) * dim) >> 32
+        li = site * dim + mu
+        assert mh.last_link == li
+        props, words = ol.device_mh_inputs(seed, step, spread, 1)
+        new = O.from_np(O.to_np(props[0, li]) @ O.to_np(u[li]))
+        p = min(max(math.exp(-ol.delta_s(u, site, mu, new, beta)), 0.0), 1.0)
+        assert abs(mh.prob_replace_last() - p) <= 1e-12 * max(p, 1e-300) + 1e-15
+        acc = O.bernoulli_accept(p, int(words[li]))
+        assert mh.has_replace_last() == acc
+        if acc:
+            u = u.copy()
+            u[li] = new
+            accepted += 1
+    assert 0 < accepted < 40
+    assert np.max(np.abs(st.link_matrix() - u)) <= 1e-12
+    with pytest.raises(ValueError):
+        lqcd.MetropolisHastingsDeltaDiagnostic(1.0)
