@@ -41,7 +41,7 @@ from ._ffi import (DRIFT_ADDITIVE, DRIFT_EXP, F32, F64, SCHEME_LEAPFROG, SCHEME_
                    LqcdError)
 
 __all__ = [
-    "Context", "LatticeCyclic", "LatticeStateDefault", "LatticeStateEFSyncDefault",
+    "Context", "LatticeCyclic", "Direction", "LatticeLink", "LatticeLinkCanonical", "LatticeStateDefault", "LatticeStateEFSyncDefault",
     "SymplecticEulerGpu", "SimulationStateLeap", "HybridMonteCarloDiagnostic", "MetropolisHastingsSweep",
     "BatchedChains", "ChainSpec", "HeatBathSweep", "OverrelaxationSweepRotation",
     "OverrelaxationSweepReverse", "HybridMethodVec", "HybridMethodCouple", "HybridMonteCarlo",
@@ -115,6 +115,98 @@ class Context:
             pass
 
 
+class Direction:
+    """``Direction<D>`` (src/lattice.rs:1110-1290): an axis index and a sign."""
+
+    __slots__ = ("_index", "_positive")
+
+    def __init__(self, index: int, is_positive: bool):
+        self._index = int(index)
+        self._positive = bool(is_positive)
+
+    def index(self) -> int:
+        return self._index
+
+    def is_positive(self) -> bool:
+        return self._positive
+
+    def is_negative(self) -> bool:
+        return not self._positive
+
+    def to_positive(self) -> "Direction":
+        return Direction(self._index, True)
+
+    def __neg__(self) -> "Direction":
+        return Direction(self._index, not self._positive)
+
+    def __eq__(self, other) -> bool:
+        return isinstance(other, Direction) and (self._index, self._positive) == (other._index, other._positive)
+
+    def __hash__(self) -> int:
+        return hash((self._index, self._positive))
+
+    def __repr__(self) -> str:
+        return f"Direction({self._index}, {self._positive})"
+
+    @staticmethod
+    def positive_directions(d: int):
+        """``positive_directions`` / ``positives_vec`` (:1139-1145, 1183)."""
+        return [Direction(i, True) for i in range(d)]
+
+    @staticmethod
+    def negative_directions(d: int):
+        return [Direction(i, False) for i in range(d)]
+
+    @staticmethod
+    def directions_vec(d: int):
+        """``directions_vec`` (:1149-1156): +0, -0, +1, -1, ..."""
+        return [Direction(i, pos) for i in range(d) for pos in (True, False)]
+
+
+class LatticeLink:
+    """``LatticeLink<D>`` (src/lattice.rs:1060-1110): a point and any direction."""
+
+    __slots__ = ("_pos", "_dir")
+
+    def __init__(self, pos, direction: Direction):
+        self._pos = tuple(int(x) for x in pos)
+        self._dir = direction
+
+    def pos(self):
+        return self._pos
+
+    def dir(self) -> Direction:
+        return self._dir
+
+    def is_dir_positive(self) -> bool:
+        return self._dir.is_positive()
+
+    def is_dir_negative(self) -> bool:
+        return self._dir.is_negative()
+
+    def __eq__(self, other) -> bool:
+        return type(other) is type(self) and (self._pos, self._dir) == (other._pos, other._dir)
+
+    def __hash__(self) -> int:
+        return hash((type(self).__name__, self._pos, self._dir))
+
+    def __repr__(self) -> str:
+        return f"{type(self).__name__}({list(self._pos)}, {self._dir!r})"
+
+
+class LatticeLinkCanonical(LatticeLink):
+    """``LatticeLinkCanonical<D>`` (src/lattice.rs:942-1058): a link with a
+    positive direction; ``new`` returns None for a negative one."""
+
+    @classmethod
+    def new(cls, pos, direction: Direction) -> Optional["LatticeLinkCanonical"]:
+        return cls(pos, direction) if direction.is_positive() else None
+
+    def to_index(self, lattice: "LatticeCyclic") -> int:
+        """position index * D + direction (:925-929): the storage index of the link."""
+        return lattice.index_of(self._pos) * lattice.dim_st() + self._dir.index()
+
+
 class LatticeCyclic:
     """``LatticeCyclic::<D>::new(size, dim)`` (src/lattice.rs:190-201).
 
@@ -164,6 +256,78 @@ class LatticeCyclic:
             idx += (int(x) % n) * stride
             stride *= n
         return idx
+
+    def point_of(self, index: int):
+        """The point of site ``index`` (the inverse of ``index_of``)."""
+        out = []
+        for n in self._extent:
+            out.append(index % n)
+            index //= n
+        return tuple(out)
+
+    def add_point_direction_n(self, point, direction: Direction, shift_number: int):
+        """``add_point_direction_n`` (src/lattice.rs:303-323): ``shift_number``
+        steps along ``direction``, periodic."""
+        p = [int(x) for x in point]
+        i = direction.index()
+        n = self._extent[i]
+        k = int(shift_number) % n
+        if direction.is_positive():
+            p[i] = (p[i] + k) % n
+        elif p[i] < k:
+            p[i] = n - (k - p[i])
+        else:
+            p[i] = (p[i] - k) % n
+        return tuple(p)
+
+    def add_point_direction(self, point, direction: Direction):
+        """``add_point_direction`` (src/lattice.rs:273-279)."""
+        return self.add_point_direction_n(point, direction, 1)
+
+    def link(self, pos, direction: Direction) -> LatticeLink:
+        """``link`` (src/lattice.rs:119-125): the position taken modulo the extents."""
+        return LatticeLink((int(x) % n for x, n in zip(pos, self._extent)), direction)
+
+    def link_canonical(self, pos, direction: Direction) -> LatticeLinkCanonical:
+        """``link_canonical`` (src/lattice.rs:84-97): a negative link becomes the
+        positive one from the neighbouring point."""
+        p = pos
+        if direction.is_negative():
+            p = self.add_point_direction(p, direction)
+        return LatticeLinkCanonical(tuple(int(x) % n for x, n in zip(p, self._extent)), direction.to_positive())
+
+    def into_canonical(self, link: LatticeLink) -> LatticeLinkCanonical:
+        """``into_canonical`` (src/lattice.rs:130-132)."""
+        return self.link_canonical(link.pos(), link.dir())
+
+    def get_points(self):
+        """``get_points`` (src/lattice.rs:155-165): every point in storage
+        order, x_0 fastest."""
+        for i in range(self.number_of_points()):
+            yield self.point_of(i)
+
+    def get_links(self):
+        """``get_links`` (src/lattice.rs:175-183): every canonical link in
+        storage order, the direction fastest."""
+        dirs = Direction.positive_directions(self._d)
+        for p in self.get_points():
+            for d in dirs:
+                yield LatticeLinkCanonical(p, d)
+
+    def has_compatible_length_links(self, links) -> bool:
+        """``has_compatible_length_links`` (src/lattice.rs:326-328), links as
+        (n_links, 18)."""
+        return len(links) == self.number_of_canonical_links_space()
+
+    def has_compatible_length_e_field(self, e_field) -> bool:
+        """``has_compatible_length_e_field`` (src/lattice.rs:331-333), E as
+        (nsite, D, 8) or (nsite * D, 8)."""
+        e = np.asarray(e_field)
+        return e.shape[0] in (self.number_of_points(), self.number_of_canonical_links_space()) and \
+            (e.size == self.number_of_canonical_links_space() * 8)
+
+    def has_compatible_length(self, links, e_field) -> bool:
+        return self.has_compatible_length_links(links) and self.has_compatible_length_e_field(e_field)
 
 
 class _DeviceLattice:
