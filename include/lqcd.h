@@ -140,6 +140,10 @@ int lqcd_hamiltonian_efield(lqcd_lattice* lat, double beta, double* out);
 /* LatticeStateWithEField::derivative_e for every site
  * (src/simulation/state.rs:1420-1448); out is [site][mu][8]. */
 int lqcd_derivative_e(lqcd_lattice* lat, double* out, size_t n_links);
+/* LatticeStateWithEField::derivative_u (src/simulation/state.rs:1407-1417)
+ * of every link: ((E U) i sqrt(2 CA)) / a with the lattice's E, out in the
+ * links' host layout [site][mu][18]. */
+int lqcd_derivative_u(lqcd_lattice* lat, double* out, size_t n_links);
 /* integrate_efield over all sites (src/integrator/mod.rs:240-254):
  * e += dt * derivative_e(U). */
 int lqcd_integrate_efield(lqcd_lattice* lat, double dt);

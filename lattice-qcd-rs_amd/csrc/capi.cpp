@@ -659,6 +659,19 @@ int lqcd_derivative_e(lqcd_lattice* L, double* out, size_t n_links)
     return force_to_host(L, out, n_links, 1.0);
 }
 
+int lqcd_derivative_u(lqcd_lattice* L, double* out, size_t n_links)
+{
+    CHECK_ARG(L && out, "null argument");
+    if (int rc = check_links(L, n_links)) return rc;
+    DeviceGuard dg(L->ctx->device);
+    DevBuf st;
+    HIP_TRY(hipMalloc(&st.p, sizeof(double) * 18 * L->nlink));
+    HIP_TRY(launch_derivative_u(L, (double*)st.p));
+    HIP_TRY(hipMemcpyAsync(out, st.p, sizeof(double) * 18 * L->nlink, hipMemcpyDeviceToHost, L->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    return LQCD_OK;
+}
+
 int lqcd_integrate_efield(lqcd_lattice* L, double dt)
 {
     CHECK_ARG(L, "lattice is null");

@@ -120,6 +120,8 @@ hipError_t launch_kick_drift_variant(const lqcd_lattice* L, int variant, const v
                                      void* E, double kick, double dt, int drift_mode);
 // out[(s*D+mu)*8+a] = scale * derivative_e^a
 hipError_t launch_force_store(const lqcd_lattice* L, const void* U, double* d_out, double scale);
+// derivative_u of every link of U[cur] with E, host layout (18 doubles per link)
+hipError_t launch_derivative_u(const lqcd_lattice* L, double* d_out);
 // E += kick * derivative_e(U)
 hipError_t launch_kick(const lqcd_lattice* L, const void* U, void* E, double kick);
 // E += kick * derivative_e(U_in); U_out = drift(U_in, E, dt)

@@ -46,6 +46,35 @@ __global__ __launch_bounds__(kBlock) void k_force_store(const vec2_t<S>* __restr
     for (int a = 0; a < 8; ++a) p[a] = oscale * (double)(fscale * f[a]);
 }
 
+// derivative_u (src/simulation/state.rs:1407-1417) of every link, double
+// arithmetic: ((E U) i sqrt(2 CA)) / a, written in the host layout
+// [site][mu][column-major 3x3][re, im]
+template <int D, typename S>
+__global__ __launch_bounds__(kBlock) void k_derivative_u(const vec2_t<S>* __restrict__ U,
+                                                         const S* __restrict__ E, Geom g, double inv_a,
+                                                         double* __restrict__ out)
+{
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int mu = blockIdx.y;
+    if (s >= g.nsite) return;
+    double e[8];
+    load_e<double, S>(E, g.npad, mu, s, e);
+    mat3<double> em, eu;
+    adjoint_to_matrix(e, em);
+    mul<false, false>(em, load_link<double, S>(U, g.npad, mu, s), eu);
+    const double c = sqrt(6.0);
+    double* p = out + (s * D + mu) * 18;
+#pragma unroll
+    for (int col = 0; col < 3; ++col)
+#pragma unroll
+        for (int row = 0; row < 3; ++row) {
+            // (x + iy) (0 + i c) = -c y + i c x, then times (1/a + 0i)
+            const cplx<double> z = eu.e[row * 3 + col];
+            p[(col * 3 + row) * 2] = (-c * z.im) * inv_a;
+            p[(col * 3 + row) * 2 + 1] = (c * z.re) * inv_a;
+        }
+}
+
 // VAR_LINK: grid (site blocks, D)
 template <int D, typename S, typename T, int DR, bool R12>
 __global__ __launch_bounds__(kBlock) void k_md_link(const vec2_t<S>* __restrict__ U_in,
@@ -158,6 +187,13 @@ static void launch_md_variant(const lqcd_lattice* L, int variant, const void* U_
 static int chosen_variant(const lqcd_lattice* L)
 {
     return (L->md_variant >= 0 && L->md_variant < VAR_COUNT) ? L->md_variant : VAR_LINK_XCD;
+}
+
+hipError_t launch_derivative_u(const lqcd_lattice* L, double* d_out)
+{
+    LQCD_DISPATCH_DS(L, k_derivative_u<D, S><<<link_grid(L), kBlock, 0, L->ctx->stream>>>(
+                            (const vec2_t<S>*)L->U[L->cur], (const S*)L->E, L->g, 1.0 / L->size, d_out));
+    return hipGetLastError();
 }
 
 hipError_t launch_force_store(const lqcd_lattice* L, const void* U, double* d_out, double scale)

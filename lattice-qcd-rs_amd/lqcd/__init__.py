@@ -673,6 +673,14 @@ class LatticeStateEFSyncDefault:
         _raise(F.lib().lqcd_derivative_e(self.handle, F.ptr(out), n))
         return out
 
+    def derivative_u(self) -> np.ndarray:
+        """``derivative_u`` (src/simulation/state.rs:1407-1417) of every link,
+        layout (nsite*D, 18) like the links."""
+        n = self.lattice().number_of_canonical_links_space()
+        out = np.empty((n, 18), dtype=np.float64)
+        _raise(F.lib().lqcd_derivative_u(self.handle, F.ptr(out), n))
+        return out
+
     def simulate_symplectic(self, integrator: "SymplecticEulerGpu", delta_t: float):
         return integrator.integrate_symplectic(self, delta_t)
 

@@ -122,6 +122,7 @@ SIGNATURES = {
     "lqcd_hamiltonian_links": (_I, [_P, _D, C.POINTER(_D)]),
     "lqcd_hamiltonian_efield": (_I, [_P, _D, C.POINTER(_D)]),
     "lqcd_derivative_e": (_I, [_P, _P, _S]),
+    "lqcd_derivative_u": (_I, [_P, _P, _S]),
     "lqcd_integrate_efield": (_I, [_P, _D]),
     "lqcd_integrate_link": (_I, [_P, _D, _I]),
     "lqcd_simulate_symplectic_n": (_I, [_P, _D, _S, _I, _I]),

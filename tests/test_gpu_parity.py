@@ -530,3 +530,19 @@ def test_metropolis_delta_diagnostic(dim, ext):
     assert np.max(np.abs(st.link_matrix() - u)) <= 1e-12
     with pytest.raises(ValueError):
         lqcd.MetropolisHastingsDeltaDiagnostic(1.0)
+
+
+@pytest.mark.parametrize("dim,ext", [(4, [4, 4, 4, 6]), (3, [6, 4, 6])])
+def test_derivative_u(dim, ext):
+    # LatticeStateWithEField::derivative_u (state.rs:1407-1417) of every link
+    # against the oracle's restatement, and the additive drift it defines
+    # (integrate_link, integrator/mod.rs:216-233): U' = U + dt derivative_u
+    announce(f"derivative_u D={dim} ext={ext}")
+    ol, u, st = make_state(dim, ext, size=0.8)
+    e = ol.random_efield(17, 0.4)
+    ef = lqcd.LatticeStateEFSyncDefault(st, e)
+    du = ef.derivative_u()
+    ref = np.array([ol.derivative_u(u, e, s, d) for s in range(ol.nsite) for d in range(dim)])
+    assert np.max(np.abs(du - ref)) <= 1e-13
+    lqcd.SymplecticEulerGpu(lqcd.DRIFT_ADDITIVE).integrate_link(ef, 0.03)
+    assert np.max(np.abs(ef.link_matrix() - (u + 0.03 * du))) <= 1e-13
