@@ -55,7 +55,7 @@ REALS_KICK_R12 = 28
 # VALU instructions per site of the fused f32 kernel (k_md_site_xcd<4, float, float, 1, R12>),
 # counted in its gfx950 assembly (hipcc -O3 --cuda-device-only -S), and the issue rate of
 # the vector ALU: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per 4 cycles per SIMD
-VALU_PER_SITE = {True: 7238, False: 6902}
+VALU_PER_SITE = {True: 7171, False: 6902}
 VALU_WAVE_INSTR_PER_S = 256 * 4 * 2.4e9 / 4
 REALS_PLAQ = 18
 REALS_EENERGY = 8

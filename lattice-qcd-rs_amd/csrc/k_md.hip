@@ -113,7 +113,11 @@ __global__ __launch_bounds__(kBlock) void k_md_site_xcd(const vec2_t<S>* __restr
     if (slab_p > 0) logical = slab_block(logical, slab_p, slab_g);
     const int64_t s = logical * kBlock + threadIdx.x;
     if (s >= g.nsite) return;
-    md_site<D, S, T, DR, R12>(U_in, U_out, E, g, s, fscale, kick, dfac);
+    // f32 two-row reads keep the own links in LDS (72 KiB at D = 4: two
+    // workgroups per CU, the occupancy the registers allow anyway)
+    constexpr bool OWN = R12 && sizeof(T) == 4;
+    __shared__ vec2_t<T> own_lds[OWN ? D * 9 * kBlock : 1];
+    md_site<D, S, T, DR, R12, OWN>(U_in, U_out, E, g, s, fscale, kick, dfac, own_lds);
 }
 
 template <int D, typename S, typename T, int MODE>

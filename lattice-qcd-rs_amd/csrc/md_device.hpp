@@ -92,13 +92,41 @@ __device__ __forceinline__ void staple_sum(const vec2_t<S>* __restrict__ U, cons
     }
 }
 
+// The site's own D links, completed to nine entries once per launch and kept
+// in LDS slots private to the thread ([mu * 9 + k][kBlock], consecutive lanes
+// in consecutive words): each is used by D - 1 upper staples, the force and
+// the drift, and re-reading it from LDS is cheaper than reloading and
+// completing it.  No barrier: a thread reads only its own slots.
+template <typename T>
+__device__ __forceinline__ mat3<T> own_get(const vec2_t<T>* own, int mu)
+{
+    mat3<T> m;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const vec2_t<T> v = own[(mu * 9 + k) * kBlock + threadIdx.x];
+        m.e[k] = {v.x, v.y};
+    }
+    return m;
+}
+template <typename T>
+__device__ __forceinline__ void own_put(vec2_t<T>* own, int mu, const mat3<T>& m)
+{
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        vec2_t<T> v;
+        v.x = m.e[k].re;
+        v.y = m.e[k].im;
+        own[(mu * 9 + k) * kBlock + threadIdx.x] = v;
+    }
+}
+
 // All D staple sums of site s.  Upper staples pairwise through
 // W = U_nu(x+mu) U_mu(x+nu)^+:  Sigma_mu += W U_nu(x)^+,
 // Sigma_nu += W^+ U_mu(x)^+.
-template <int D, typename S, typename T, bool R12 = false>
+template <int D, typename S, typename T, bool R12 = false, bool OWN = false>
 __device__ __forceinline__ void staple_sums_site(const vec2_t<S>* __restrict__ U, const Geom& g,
                                                  int64_t s, const Neighbours<D>& nb,
-                                                 mat3<T> (&sig)[D])
+                                                 mat3<T> (&sig)[D], const vec2_t<T>* own = nullptr)
 {
 #pragma unroll
     for (int mu = 0; mu < D; ++mu) set_zero(sig[mu]);
@@ -118,12 +146,12 @@ __device__ __forceinline__ void staple_sums_site(const vec2_t<S>* __restrict__ U
                     complete_su3(w);
                 }
                 {
-                    const mat3<T> c = load_link_r12<T, S>(U, g.npad, nu, s);
+                    const mat3<T> c = OWN ? own_get<T>(own, nu) : load_link_r12<T, S>(U, g.npad, nu, s);
                     mul_rows01<false, true>(w, c, st);
                     add_su3(st, sig[mu]);
                 }
                 {
-                    const mat3<T> c = load_link_r12<T, S>(U, g.npad, mu, s);
+                    const mat3<T> c = OWN ? own_get<T>(own, mu) : load_link_r12<T, S>(U, g.npad, mu, s);
                     mul_rows01<true, true>(w, c, st);
                     add_su3(st, sig[nu]);
                 }
@@ -241,20 +269,25 @@ __device__ __forceinline__ void md_link(const vec2_t<S>* __restrict__ U_in, vec2
 }
 
 // all D links of site s, sharing the staple loads
-template <int D, typename S, typename T, int DR, bool R12 = false>
+template <int D, typename S, typename T, int DR, bool R12 = false, bool OWN = false>
 __device__ __forceinline__ void md_site(const vec2_t<S>* __restrict__ U_in, vec2_t<S>* __restrict__ U_out,
                                         S* __restrict__ E, const Geom& g, int64_t s, T fscale, T kick,
-                                        T dfac)
+                                        T dfac, vec2_t<T>* own = nullptr)
 {
     const Neighbours<D> nb = neighbours<D>(g, s);
+    // OWN (two-row reads, f32): the own links into the thread's LDS slots
+    if constexpr (OWN) {
+#pragma unroll
+        for (int mu = 0; mu < D; ++mu) own_put<T>(own, mu, load_link_r12<T, S>(U_in, g.npad, mu, s));
+    }
     // phase 1: the D forces (the staple sums die here)
     T f[D][8];
     {
         mat3<T> sig[D];
-        staple_sums_site<D, S, T, R12>(U_in, g, s, nb, sig);
+        staple_sums_site<D, S, T, R12, OWN>(U_in, g, s, nb, sig, own);
 #pragma unroll
         for (int mu = 0; mu < D; ++mu) {
-            const mat3<T> u = load_link_sel<R12, T, S>(U_in, g.npad, mu, s);
+            const mat3<T> u = OWN ? own_get<T>(own, mu) : load_link_sel<R12, T, S>(U_in, g.npad, mu, s);
             mat3<T> m;
             mul<false, false>(u, sig[mu], m);
             im_trace_generators(m, f[mu]);
@@ -273,7 +306,7 @@ __device__ __forceinline__ void md_site(const vec2_t<S>* __restrict__ U_in, vec2
     if (DR >= 0) {
 #pragma unroll
         for (int mu = 0; mu < D; ++mu) {
-            const mat3<T> u = load_link_sel<R12, T, S>(U_in, g.npad, mu, s);
+            const mat3<T> u = OWN ? own_get<T>(own, mu) : load_link_sel<R12, T, S>(U_in, g.npad, mu, s);
             mat3<T> un;
             drift_link<T, DR>(u, e[mu], dfac, un);
             store_link<T, S, kCacheStream>(U_out, g.npad, mu, s, un);
