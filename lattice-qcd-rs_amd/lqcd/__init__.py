@@ -559,6 +559,45 @@ class LatticeStateEFSyncDefault:
     def new_e_cold(cls, lattice_state: LatticeStateDefault):
         return cls(lattice_state, None)
 
+    @classmethod
+    def new_cold(cls, size: float, beta: float, number_of_points, d: int = 4, precision: int = F64,
+                 ctx: Optional[Context] = None):
+        """``new_cold`` (src/simulation/state.rs:1221-1231): identity links, zero E."""
+        return cls(LatticeStateDefault.new_cold(size, beta, number_of_points, d, precision, ctx), None)
+
+    @classmethod
+    def new_determinist_cold_e_hot_link(cls, size: float, beta: float, number_of_points, seed: int,
+                                        d: int = 4, precision: int = F64, ctx: Optional[Context] = None):
+        """``new_determinist_cold_e_hot_link`` (src/simulation/state.rs:1197-1211):
+        random SU(3) links drawn on the device from ``seed``, zero E."""
+        st = LatticeStateDefault.new_determinist(size, beta, number_of_points, seed, d, precision, ctx)
+        return cls(st, None)
+
+    @classmethod
+    def new_determinist(cls, size: float, beta: float, number_of_points, seed: int, distribution,
+                        d: int = 4, precision: int = F64, ctx: Optional[Context] = None):
+        """``new_determinist`` (src/simulation/state.rs:1175-1189): random SU(3)
+        links drawn on the device from ``seed`` and E components drawn on the
+        host from ``distribution`` (a callable taking the shape and returning
+        float64 samples, the role of the reference's ``rand_distr`` argument);
+        no Gauss projection, as in the reference."""
+        st = LatticeStateDefault.new_determinist(size, beta, number_of_points, seed, d, precision, ctx)
+        n = st.lattice().number_of_canonical_links_space()
+        e = np.asarray(distribution((n, 8)), dtype=np.float64)
+        return cls(st, e)
+
+    @classmethod
+    def new_random_e(cls, lattice: LatticeCyclic, beta: float, link_matrix, seed: int,
+                     precision: int = F64, ctx: Optional[Context] = None):
+        """``LatticeStateWithEFieldNew::new_random_e`` (src/simulation/state.rs:262-277):
+        the given links, E ~ N(0, 0.5 / beta) drawn on the device and projected
+        on Gauss's law."""
+        if not (beta > 0) or math.isinf(beta):
+            # Normal::new(0, 0.5 / beta) fails for these
+            raise StateInitializationError("0.5 / beta is not a valid standard deviation")
+        st = LatticeStateDefault(lattice, beta, link_matrix, precision, ctx)
+        return cls.new_random_e_state(st, seed)
+
     # serde interchange (state.rs:1046-1062): { e_field, t, lattice_state }
     def write(self, path: str, format: int = F.FORMAT_BINCODE) -> None:
         _raise(F.lib().lqcd_state_ef_write(self.handle, self._state.beta(), C.c_uint64(self._t),

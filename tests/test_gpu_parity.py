@@ -546,3 +546,31 @@ def test_derivative_u(dim, ext):
     assert np.max(np.abs(du - ref)) <= 1e-13
     lqcd.SymplecticEulerGpu(lqcd.DRIFT_ADDITIVE).integrate_link(ef, 0.03)
     assert np.max(np.abs(ef.link_matrix() - (u + 0.03 * du))) <= 1e-13
+
+
+def test_efsync_constructors():
+    # LatticeStateEFSyncDefault's constructors (state.rs:262-277, 1175-1231)
+    announce("EFSync constructors")
+    a = lqcd.LatticeStateEFSyncDefault.new_cold(1.0, 6.0, 4, 3)
+    eye = O.from_np(np.eye(3, dtype=complex))
+    assert np.all(a.e_field() == 0) and np.all(a.link_matrix() == eye)
+    b = lqcd.LatticeStateEFSyncDefault.new_determinist_cold_e_hot_link(1.0, 6.0, 4, 5, 3)
+    ref = lqcd.LatticeStateDefault.new_determinist(1.0, 6.0, 4, 5, 3)
+    assert np.all(b.e_field() == 0) and np.array_equal(b.link_matrix(), ref.link_matrix())
+    # the reference's doctest (state.rs:1165-1173): the same seeds give the same state
+    states = []
+    for _ in range(2):
+        rng = np.random.default_rng(0)
+        states.append(lqcd.LatticeStateEFSyncDefault.new_determinist(
+            1.0, 1.0, 4, 0, lambda shape: rng.uniform(-1.0, 1.0, shape), 4))
+    assert np.array_equal(states[0].e_field(), states[1].e_field())
+    assert np.array_equal(states[0].link_matrix(), states[1].link_matrix())
+    assert np.all(np.abs(states[0].e_field()) < 1.0) and np.any(states[0].e_field() != 0)
+    lat = lqcd.LatticeCyclic(1.0, 4, 3)
+    u = ref.link_matrix()
+    c = lqcd.LatticeStateEFSyncDefault.new_random_e(lat, 6.0, u, 7)
+    assert np.array_equal(c.link_matrix(), u) and np.any(c.e_field() != 0)
+    # projected: below project_to_gauss's stopping threshold N * 4 * 8 * 10 * eps
+    assert c.gauss_sum_div() <= 4 ** 3 * 4 * 8 * 10 * 2.220446049250313e-16
+    with pytest.raises(lqcd.StateInitializationError):
+        lqcd.LatticeStateEFSyncDefault.new_random_e(lat, 0.0, u, 7)
