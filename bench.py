@@ -26,6 +26,8 @@ Other workloads (--workload), one JSON line each, same keys:
            50 leapfrog steps dt = 0.02, plaquette after every trajectory.
   mh3d     D = 3 128^3 FP64 checkerboard Metropolis sweeps (spread 0.1,
            beta = 6.0), proposals and accept draws generated on the device.
+  sweeps   D = 4 32^4 FP64 quenched update: one HeatBathSweep, normalize_link_matrices,
+           four OverrelaxationSweepRotation per step (checkerboard, device streams).
   stats    statistics::mean_and_variance (two-pass) of 2^27 f64 values resident
            in device memory.
   small    8^4 FP64: plaquette, HMC trajectories of 20 steps dt = 0.01
@@ -79,7 +81,8 @@ def parse():
     p.add_argument("--copy-only", action="store_true",
                    help="only run the 1 GiB streaming copy (byte-counter calibration)")
     p.add_argument("--workload", default="hmc32",
-                   choices=["hmc32", "batched", "large", "mh3d", "small", "fields", "state", "stats"])
+                   choices=["hmc32", "batched", "large", "mh3d", "small", "fields", "state", "stats",
+                            "sweeps"])
     p.add_argument("--precision-trajectories", type=int, default=3,
                    help="trajectories of the FP32-mixed against FP64 error table (0 = skip)")
     p.add_argument("--gauss-checks", type=int, default=4000,
@@ -504,6 +507,71 @@ def bench_fields(args, lqcd):
         cfg, roofline_from(rate, "k_clover_fields<4,float,1> (clover, F_mu_nu)"), cpu)
 
 
+def bench_sweeps(args, lqcd):
+    """A quenched update of the 32^4 D=4 f64 lattice: HeatBathSweep
+    (heat_bath.rs:73-123), normalize_link_matrices, then four
+    OverrelaxationSweepRotation (overrelaxation.rs:66-91), as the reference's
+    HybridMethodVec would chain them; checkerboard class launches."""
+    F = lqcd._ffi
+    L, beta = args.extent, args.beta
+    st = lqcd.LatticeStateDefault.new_determinist(1.0, beta, L, args.seed, 4, lqcd.F64)
+    hb = lqcd.HeatBathSweep(args.seed)
+    orx = lqcd.OverrelaxationSweepRotation()
+    ctx = st._dev.ctx
+
+    def step():
+        st.monte_carlo_step(hb)
+        st.normalize_link_matrices()
+        for _ in range(4):
+            st.monte_carlo_step(orx)
+
+    for _ in range(args.warmup):
+        step()
+    kinds = (F.KERNEL_HEAT_BATH, F.KERNEL_OVERRELAX, F.KERNEL_REUNIT)
+    ctx.profile_enable(sum(1 << k for k in kinds))
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.synchronize()
+    seconds = time.perf_counter() - t0
+    nlink = st.lattice().number_of_canonical_links_space()
+    # one class launch: every link read once for the staples (144 B each), the
+    # class's nlink / 8 links written (and the heat bath's per-link flag)
+    hb_rate = kernel_rate(ctx, F.KERNEL_HEAT_BATH, nlink * 144 + nlink // 8 * (144 + 4))
+    or_rate = kernel_rate(ctx, F.KERNEL_OVERRELAX, nlink * 144 + nlink // 8 * 144)
+    re_rate = kernel_rate(ctx, F.KERNEL_REUNIT, nlink * REALS_REUNIT * 8)
+    ctx.profile_enable(0)
+    value = nlink * 5 * args.steps / seconds
+    cfg = {"workload": ("32^4 D=4 FP64 quenched update per step: HeatBathSweep, normalize_link_matrices, "
+                        "4 OverrelaxationSweepRotation (checkerboard), beta 6.0, hot start"),
+           "lattice": [L] * 4, "links": nlink, "beta": beta}
+    cpu = None
+    if args.cpu_md_steps > 0:
+        import oracle as O
+        ol = O.OracleLattice(4, 16)
+        u = ol.hot_start(args.seed)
+        k, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < min(args.cpu_seconds, 10.0):
+            u, _ = ol.heat_bath_sweep(u, beta, args.seed, k)
+            u = ol.normalize_link_matrices(u)
+            for _ in range(4):
+                u = ol.overrelaxation_sweep(u)
+            k += 1
+        t = time.perf_counter() - t0
+        cpu = {"value": ol.nlink * 5 * k / t, "unit": "link updates/s", "cores": 1, "kind": "port",
+               "sample": (f"{k} steps (heat bath, normalisation, 4 overrelaxation sweeps) of the oracle "
+                          f"on 16^4, single-threaded sweeps as in the reference, {t:.1f} s")}
+    dominant, name = ((hb_rate, "k_heat_bath<4,double> (one checkerboard class)")
+                      if hb_rate["avg_us"] * 8 >= or_rate["avg_us"] * 32
+                      else (or_rate, "k_overrelax<4,double,false> (one checkerboard class)"))
+    return contract_line(
+        args, "link updates/s, 32^4 FP64 heat bath + overrelaxation", value, "link updates/s", args.steps,
+        seconds, "f64", cfg, roofline_from(dominant, name), cpu,
+        kernels={"heat_bath": hb_rate, "overrelaxation": or_rate, "reunitarize": re_rate},
+        sweeps_per_s=round(5 * args.steps / seconds, 3))
+
+
 def bench_stats(args, lqcd):
     """statistics::mean_and_variance (statistics/mod.rs:375-399, two passes:
     the sum, then sum (x - mean)^2) of 2^27 f64 values resident in device
@@ -652,7 +720,7 @@ def main():
     if args.workload != "hmc32":
         run = {"batched": bench_batched, "large": bench_large, "mh3d": bench_mh3d,
                "small": bench_small, "fields": bench_fields,
-               "state": bench_state, "stats": bench_stats}[args.workload]
+               "state": bench_state, "stats": bench_stats, "sweeps": bench_sweeps}[args.workload]
         print(json.dumps(run(args, lqcd)))
         return
 

@@ -440,7 +440,9 @@ typedef enum {
     LQCD_KERNEL_MOMENTA = 7,
     LQCD_KERNEL_FORCE = 8,
     LQCD_KERNEL_FIELDS = 9,
-    LQCD_KERNEL_STATS = 10
+    LQCD_KERNEL_STATS = 10,
+    LQCD_KERNEL_HEAT_BATH = 11,
+    LQCD_KERNEL_OVERRELAX = 12
 } lqcd_kernel_kind;
 
 /* ---------------------------------------------------------- statistics */

@@ -85,7 +85,9 @@ enum ProfileKind {
     PROF_FORCE_STORE = 8,
     PROF_FIELDS = 9,
     PROF_STATS = 10,
-    PROF_NKINDS = 11
+    PROF_HEAT_BATH = 11,
+    PROF_OVERRELAX = 12,
+    PROF_NKINDS = 13
 };
 
 // ---- fields (k_fields.hip) ----

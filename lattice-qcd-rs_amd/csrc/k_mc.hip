@@ -533,7 +533,7 @@ hipError_t launch_heat_bath_class(const lqcd_lattice* L, void* U, int mu, int pa
 {
     const unsigned grid = grid_for(L->g.nsite / 2);
     lqcd_prof_event ev;
-    prof_begin(L->ctx, PROF_METROPOLIS, &ev);
+    prof_begin(L->ctx, PROF_HEAT_BATH, &ev);
     LQCD_DISPATCH_DS(L, k_heat_bath<D, S><<<grid, kBlock, 0, L->ctx->stream>>>(
                             (vec2_t<S>*)U, L->g, mu, parity, beta, seed, counter, L->mh_p, L->mh_acc));
     prof_end(L->ctx, &ev);
@@ -543,12 +543,15 @@ hipError_t launch_heat_bath_class(const lqcd_lattice* L, void* U, int mu, int pa
 hipError_t launch_overrelax_class(const lqcd_lattice* L, void* U, int mu, int parity, bool reverse)
 {
     const unsigned grid = grid_for(L->g.nsite / 2);
+    lqcd_prof_event ev;
+    prof_begin(L->ctx, PROF_OVERRELAX, &ev);
     if (reverse)
         LQCD_DISPATCH_DS(L, k_overrelax<D, S, true><<<grid, kBlock, 0, L->ctx->stream>>>((vec2_t<S>*)U, L->g,
                                                                                          mu, parity));
     else
         LQCD_DISPATCH_DS(L, k_overrelax<D, S, false><<<grid, kBlock, 0, L->ctx->stream>>>((vec2_t<S>*)U, L->g,
                                                                                           mu, parity));
+    prof_end(L->ctx, &ev);
     return hipGetLastError();
 }
 

@@ -43,6 +43,8 @@ KERNEL_MOMENTA = 7
 KERNEL_FORCE = 8
 KERNEL_FIELDS = 9
 KERNEL_STATS = 10
+KERNEL_HEAT_BATH = 11
+KERNEL_OVERRELAX = 12
 STATS_SEQUENTIAL, STATS_PARALLEL = 0, 1
 FIELD_CLOVER, FIELD_F_MU_NU, FIELD_MAGNETIC = 0, 1, 2
 
