@@ -148,6 +148,68 @@ int links_su3(lqcd_lattice* L, int64_t updates, bool* ok)
     return LQCD_OK;
 }
 
+// The n fused kick+drift launches of a leapfrog integration from U[cur]
+// (kick dt/2 then dt, drift dt, buffers alternating w1, w2), replayed from a
+// graph captured on first use.  Graph replays start each kernel as the last
+// one ends; eager launches leave about 10 us between them.  One graph per
+// starting buffer; all are captured again when a parameter they bake in
+// changes.  With profiling on, one event pair spans the replay and counts n
+// launches of the fused kernel.
+static int leapfrog_graph_replay(lqcd_lattice* L, double dt, size_t n, int drift)
+{
+    if (L->mg_steps != n || L->mg_dt != dt || L->mg_drift != drift || L->mg_variant != L->md_variant ||
+        L->mg_r12 != L->md_r12) {
+        for (auto& ge : L->md_graph)
+            if (ge) {
+                (void)hipGraphExecDestroy(ge);
+                ge = nullptr;
+            }
+        L->mg_steps = n;
+        L->mg_dt = dt;
+        L->mg_drift = drift;
+        L->mg_variant = L->md_variant;
+        L->mg_r12 = L->md_r12;
+    }
+    hipGraphExec_t& exec = L->md_graph[L->cur];
+    hipStream_t st = L->ctx->stream;
+    if (!exec) {
+        int w1, w2;
+        other_buffers(L->cur, &w1, &w2);
+        // timing events stay out of the graph: it holds kernel nodes only
+        const bool profiling = L->ctx->profiling;
+        L->ctx->profiling = false;
+        HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipError_t le = hipSuccess;
+        int src = L->cur, dst = w1;
+        for (size_t k = 0; k < n && le == hipSuccess; ++k) {
+            le = launch_kick_drift(L, L->U[src], L->U[dst], L->E, k == 0 ? dt / 2.0 : dt, dt, drift);
+            src = dst;
+            dst = (dst == w1) ? w2 : w1;
+        }
+        hipGraph_t graph = nullptr;
+        const hipError_t ce = hipStreamEndCapture(st, &graph);
+        L->ctx->profiling = profiling;
+        if (le != hipSuccess || ce != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            (void)hipGetLastError();
+            return fail(LQCD_ERR_DEVICE, std::string("MD capture: ") +
+                                             hipGetErrorString(le != hipSuccess ? le : ce));
+        }
+        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) {
+            exec = nullptr;
+            return fail(LQCD_ERR_DEVICE, std::string("MD graph instantiate: ") + hipGetErrorString(ie));
+        }
+    }
+    lqcd_prof_event ev;
+    prof_begin(L->ctx, PROF_KICK_DRIFT, &ev);
+    ev.launches = (int)n;
+    HIP_TRY(hipGraphLaunch(exec, st));
+    prof_end(L->ctx, &ev);
+    return LQCD_OK;
+}
+
 // n steps of the chosen scheme starting from U[cur]; *last gets the buffer
 // holding the final links.  U[cur] is left untouched.  An integration with
 // the exp drift keeps the links in SU(3) (up to rounding), so when the
@@ -176,10 +238,15 @@ int integrate_steps(lqcd_lattice* L, double dt, size_t n, int drift, int scheme,
             dst = (dst == w1) ? w2 : w1;
         }
     } else {
-        for (size_t k = 0; k < n; ++k) {
-            HIP_TRY(launch_kick_drift(L, L->U[src], L->U[dst], L->E, k == 0 ? dt / 2.0 : dt, dt, drift));
-            src = dst;
-            dst = (dst == w1) ? w2 : w1;
+        if (L->md_graphs) {
+            if (int rc = leapfrog_graph_replay(L, dt, n, drift)) return rc;
+            src = (n % 2) ? w1 : (n == 0 ? src : w2);
+        } else {
+            for (size_t k = 0; k < n; ++k) {
+                HIP_TRY(launch_kick_drift(L, L->U[src], L->U[dst], L->E, k == 0 ? dt / 2.0 : dt, dt, drift));
+                src = dst;
+                dst = (dst == w1) ? w2 : w1;
+            }
         }
         HIP_TRY(launch_kick(L, L->U[src], L->E, dt / 2.0));
     }
@@ -203,6 +270,7 @@ namespace lqcd {
 void prof_begin(lqcd_ctx* ctx, int kind, lqcd_prof_event* ev)
 {
     ev->kind = -1;
+    ev->launches = 1;
     if (!ctx->profiling || !(ctx->profile_mask & (1u << kind))) return;
     lqcd_prof_event e;
     if (!ctx->event_pool.empty()) {
@@ -373,6 +441,8 @@ int lqcd_lattice_destroy(lqcd_lattice* L)
     if (L->gauss_g) (void)hipFree(L->gauss_g);
     for (int k = 0; k < 2; ++k)
         if (L->gauss_e[k]) (void)hipFree(L->gauss_e[k]);
+    for (auto& ge : L->md_graph)
+        if (ge) (void)hipGraphExecDestroy(ge);
     delete L;
     return LQCD_OK;
 }
@@ -1070,6 +1140,14 @@ int lqcd_md_reconstruct_set(lqcd_lattice* L, int mode)
     return LQCD_OK;
 }
 
+int lqcd_md_graph_set(lqcd_lattice* L, int enable)
+{
+    CHECK_ARG(L, "lattice is null");
+    CHECK_ARG(enable == 0 || enable == 1, "enable must be 0 or 1");
+    L->md_graphs = enable == 1;
+    return LQCD_OK;
+}
+
 int lqcd_md_reconstruct_get(const lqcd_lattice* L, int* mode, int* last_used)
 {
     CHECK_ARG(L && mode && last_used, "null argument");
@@ -1112,7 +1190,7 @@ int lqcd_profile_read(lqcd_ctx* ctx, int kind, size_t* count, double* total_ms)
         float ms = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms, e.start, e.stop));
         t += ms;
-        ++c;
+        c += (size_t)e.launches;
     }
     *count = c;
     *total_ms = t;

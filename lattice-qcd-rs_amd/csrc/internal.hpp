@@ -14,6 +14,7 @@
 struct lqcd_prof_event {
     int kind;
     hipEvent_t start, stop;
+    int launches = 1;   // kernels between start and stop (a graph replay holds several)
 };
 
 struct lqcd_ctx {
@@ -46,6 +47,14 @@ struct lqcd_lattice {
     int md_r12_mode = 1;           // 1: exp-drift integrations read SU(3) links as two rows
     bool md_r12 = false;           // set for the duration of one integration that does
     bool md_r12_last = false;      // whether the last integration did
+    // leapfrog MD launches replayed from captured graphs, one per starting
+    // buffer; the other fields are the launch parameters the graphs bake in
+    bool md_graphs = true;
+    hipGraphExec_t md_graph[3] = {nullptr, nullptr, nullptr};
+    size_t mg_steps = 0;
+    double mg_dt = 0.0;
+    int mg_drift = -1, mg_variant = -1;
+    bool mg_r12 = false;
     // what is known of the distance of U[cur] from SU(3): >= 0 within
     // r12_distance, counting the SU(3)-preserving updates since that was
     // established; -1 unknown; -2 known to be off SU(3)

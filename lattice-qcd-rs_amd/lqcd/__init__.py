@@ -394,6 +394,11 @@ class LatticeStateDefault:
         _raise(F.lib().lqcd_links_random(st.handle, C.c_uint64(seed)))
         return st
 
+    def set_md_graph(self, enable: bool) -> None:
+        """Replay leapfrog MD launches from a captured graph (default) or launch
+        them one by one (``lqcd_md_graph_set``); the results are the same."""
+        _raise(F.lib().lqcd_md_graph_set(self.handle, 1 if enable else 0))
+
     def set_md_reconstruct(self, mode: int) -> None:
         """1 (default): exp-drift integrations read SU(3) links as two rows;
         0: always nine entries."""

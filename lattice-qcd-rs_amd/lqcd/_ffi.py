@@ -152,6 +152,7 @@ SIGNATURES = {
     "lqcd_stats_mean_with_error": (_I, [_P, _P, _S, _I, _P]),
     "lqcd_stats_covariance": (_I, [_P, _P, _S, _P, _S, _I, _I, _P, C.POINTER(C.c_int)]),
     "lqcd_md_reconstruct_set": (_I, [_P, _I]),
+    "lqcd_md_graph_set": (_I, [_P, _I]),
     "lqcd_batch_reconstruct_set": (_I, [_P, _I]),
     "lqcd_batch_reconstruct_get": (_I, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "lqcd_md_reconstruct_get": (_I, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
