@@ -347,10 +347,10 @@ int lqcd_md_variant_get(const lqcd_lattice* lat, int* variant);
  * SU(3)-preserving updates per link.  *last_used tells whether the last
  * integration or device Metropolis sweep read two rows. */
 int lqcd_md_reconstruct_set(lqcd_lattice* lat, int mode);
-/* 1 (default): the fused kick+drift launches of a leapfrog integration
- * (LQCD_SCHEME_LEAPFROG) are replayed from a HIP graph captured on first use
- * for the starting buffer and parameters; 0: launched one by one.  Same
- * results; the graph removes the ~10 us the device leaves between eager
+/* 1 (default): the MD launches of an integration are replayed from a HIP
+ * graph captured on first use for the starting buffer and parameters (the
+ * reference scheme only while profiling is off); 0: launched one by one.
+ * Same results; the graph removes the ~10 us the device leaves between eager
  * launches.  No counterpart in the reference. */
 int lqcd_md_graph_set(lqcd_lattice* lat, int enable);
 int lqcd_md_reconstruct_get(const lqcd_lattice* lat, int* mode, int* last_used);

@@ -148,17 +148,20 @@ int links_su3(lqcd_lattice* L, int64_t updates, bool* ok)
     return LQCD_OK;
 }
 
-// The n fused kick+drift launches of a leapfrog integration from U[cur]
-// (kick dt/2 then dt, drift dt, buffers alternating w1, w2), replayed from a
-// graph captured on first use.  Graph replays start each kernel as the last
-// one ends; eager launches leave about 10 us between them.  One graph per
-// starting buffer; all are captured again when a parameter they bake in
-// changes.  With profiling on, one event pair spans the replay and counts n
-// launches of the fused kernel.
-static int leapfrog_graph_replay(lqcd_lattice* L, double dt, size_t n, int drift)
+// The MD launches of an integration from U[cur], buffers alternating w1, w2,
+// replayed from a graph captured on first use.  Graph replays start each
+// kernel as the last one ends; eager launches leave about 10 us between them.
+//   LEAPFROG: the n fused kick+drift launches (kick dt/2 then dt, drift dt);
+//             the final half kick stays outside.  With profiling on, one
+//             event pair spans the replay and counts n fused launches.
+//   REFERENCE: per step a fused kick(dt/2)+drift(dt) and a kick(dt/2); used
+//             only with profiling off (the replay mixes two kernel kinds).
+// One graph per starting buffer; all are captured again when a parameter
+// they bake in changes.
+static int md_graph_replay(lqcd_lattice* L, double dt, size_t n, int drift, int scheme)
 {
     if (L->mg_steps != n || L->mg_dt != dt || L->mg_drift != drift || L->mg_variant != L->md_variant ||
-        L->mg_r12 != L->md_r12) {
+        L->mg_r12 != L->md_r12 || L->mg_scheme != scheme) {
         for (auto& ge : L->md_graph)
             if (ge) {
                 (void)hipGraphExecDestroy(ge);
@@ -169,6 +172,7 @@ static int leapfrog_graph_replay(lqcd_lattice* L, double dt, size_t n, int drift
         L->mg_drift = drift;
         L->mg_variant = L->md_variant;
         L->mg_r12 = L->md_r12;
+        L->mg_scheme = scheme;
     }
     hipGraphExec_t& exec = L->md_graph[L->cur];
     hipStream_t st = L->ctx->stream;
@@ -182,7 +186,12 @@ static int leapfrog_graph_replay(lqcd_lattice* L, double dt, size_t n, int drift
         hipError_t le = hipSuccess;
         int src = L->cur, dst = w1;
         for (size_t k = 0; k < n && le == hipSuccess; ++k) {
-            le = launch_kick_drift(L, L->U[src], L->U[dst], L->E, k == 0 ? dt / 2.0 : dt, dt, drift);
+            if (scheme == LQCD_SCHEME_REFERENCE) {
+                le = launch_kick_drift(L, L->U[src], L->U[dst], L->E, dt / 2.0, dt, drift);
+                if (le == hipSuccess) le = launch_kick(L, L->U[dst], L->E, dt / 2.0);
+            } else {
+                le = launch_kick_drift(L, L->U[src], L->U[dst], L->E, k == 0 ? dt / 2.0 : dt, dt, drift);
+            }
             src = dst;
             dst = (dst == w1) ? w2 : w1;
         }
@@ -210,6 +219,14 @@ static int leapfrog_graph_replay(lqcd_lattice* L, double dt, size_t n, int drift
     return LQCD_OK;
 }
 
+// the buffer holding the links after n drifts from U[cur]
+static int buffer_after(const lqcd_lattice* L, size_t n)
+{
+    int w1, w2;
+    other_buffers(L->cur, &w1, &w2);
+    return n == 0 ? L->cur : ((n % 2) ? w1 : w2);
+}
+
 // n steps of the chosen scheme starting from U[cur]; *last gets the buffer
 // holding the final links.  U[cur] is left untouched.  An integration with
 // the exp drift keeps the links in SU(3) (up to rounding), so when the
@@ -231,16 +248,21 @@ int integrate_steps(lqcd_lattice* L, double dt, size_t n, int drift, int scheme,
     other_buffers(L->cur, &w1, &w2);
     int src = L->cur, dst = w1;
     if (scheme == LQCD_SCHEME_REFERENCE) {
-        for (size_t k = 0; k < n; ++k) {
-            HIP_TRY(launch_kick_drift(L, L->U[src], L->U[dst], L->E, dt / 2.0, dt, drift));
-            HIP_TRY(launch_kick(L, L->U[dst], L->E, dt / 2.0));
-            src = dst;
-            dst = (dst == w1) ? w2 : w1;
+        if (L->md_graphs && !L->ctx->profiling) {
+            if (int rc = md_graph_replay(L, dt, n, drift, scheme)) return rc;
+            src = buffer_after(L, n);
+        } else {
+            for (size_t k = 0; k < n; ++k) {
+                HIP_TRY(launch_kick_drift(L, L->U[src], L->U[dst], L->E, dt / 2.0, dt, drift));
+                HIP_TRY(launch_kick(L, L->U[dst], L->E, dt / 2.0));
+                src = dst;
+                dst = (dst == w1) ? w2 : w1;
+            }
         }
     } else {
         if (L->md_graphs) {
-            if (int rc = leapfrog_graph_replay(L, dt, n, drift)) return rc;
-            src = (n % 2) ? w1 : (n == 0 ? src : w2);
+            if (int rc = md_graph_replay(L, dt, n, drift, scheme)) return rc;
+            src = buffer_after(L, n);
         } else {
             for (size_t k = 0; k < n; ++k) {
                 HIP_TRY(launch_kick_drift(L, L->U[src], L->U[dst], L->E, k == 0 ? dt / 2.0 : dt, dt, drift));

@@ -53,7 +53,7 @@ struct lqcd_lattice {
     hipGraphExec_t md_graph[3] = {nullptr, nullptr, nullptr};
     size_t mg_steps = 0;
     double mg_dt = 0.0;
-    int mg_drift = -1, mg_variant = -1;
+    int mg_drift = -1, mg_variant = -1, mg_scheme = -1;
     bool mg_r12 = false;
     // what is known of the distance of U[cur] from SU(3): >= 0 within
     // r12_distance, counting the SU(3)-preserving updates since that was

@@ -29,13 +29,18 @@ def pair(precision, seed=3):
     return out
 
 
-@pytest.mark.parametrize("precision", [lqcd.F64, lqcd.F32])
-def test_graph_replays_match_eager(precision):
-    announce(f"MD graph vs eager precision={precision}")
+@pytest.mark.parametrize("precision,drift,scheme", [
+    (lqcd.F64, lqcd.DRIFT_EXP, lqcd.SCHEME_LEAPFROG),
+    (lqcd.F32, lqcd.DRIFT_EXP, lqcd.SCHEME_LEAPFROG),
+    (lqcd.F64, lqcd.DRIFT_ADDITIVE, lqcd.SCHEME_REFERENCE),
+    (lqcd.F32, lqcd.DRIFT_EXP, lqcd.SCHEME_REFERENCE),
+])
+def test_graph_replays_match_eager(precision, drift, scheme):
+    announce(f"MD graph vs eager precision={precision} drift={drift} scheme={scheme}")
     a, b = pair(precision)
     efa = lqcd.LatticeStateEFSyncDefault.new_random_e_state(a, 5, 0, project=False)
     efb = lqcd.LatticeStateEFSyncDefault.new_random_e_state(b, 5, 0, project=False)
-    integ = lqcd.SymplecticEulerGpu(lqcd.DRIFT_EXP, lqcd.SCHEME_LEAPFROG)
+    integ = lqcd.SymplecticEulerGpu(drift, scheme)
     for dt, n in ((0.02, 7), (0.02, 7), (0.02, 7), (0.015, 4), (0.02, 7)):
         efa.simulate_symplectic_n(integ, dt, n)
         efb.simulate_symplectic_n(integ, dt, n)
@@ -56,7 +61,7 @@ def test_graph_replays_match_eager(precision):
         assert (sa.h_old, sa.h_new, sa.accepted) == (sb.h_old, sb.h_new, sb.accepted)
         assert np.array_equal(a.link_matrix(), b.link_matrix())
         accepted += sa.accepted
-    assert accepted >= 2
+    assert accepted >= 1
 
 
 def test_graph_profiling_counts_launches():
