@@ -275,6 +275,13 @@ int lqcd_overrelaxation_reverse_sweep(lqcd_lattice* lat);
  * is clover(x, i, j), 18 doubles (column-major 3x3 complex, the layout of one
  * link).  n = nsite * D * D * 18.  Double arithmetic on either storage type. */
 int lqcd_clover_field(lqcd_lattice* lat, double* out, size_t n);
+/* LinkMatrix::sij (which 0) or pij (which 1) (src/field.rs:743-771) at every
+ * site for the signed directions (dir_i, positive_i), (dir_j, positive_j):
+ * S_ij(x) = U_j(x) U_i(x+j) U_j(x+i)^+, P_ij(x) = U_i(x) S_ij(x)^+ with
+ * U_{-d}(y) = U_d(y-d)^+; out[site] 18 doubles (column-major 3x3 complex),
+ * n = nsite * 18. */
+int lqcd_plaquette_field(lqcd_lattice* lat, int which, int dir_i, int positive_i, int dir_j, int positive_j,
+                         double* out, size_t n);
 /* LinkMatrix::f_mu_nu (src/field.rs:822-834), (clover(i,j) - clover(j,i)) /
  * (8 a^2), same layout and size as lqcd_clover_field. */
 int lqcd_f_mu_nu_field(lqcd_lattice* lat, double* out, size_t n);

@@ -620,6 +620,25 @@ static int clover_fields(lqcd_lattice* L, int mode, double* out, size_t n)
 
 int lqcd_clover_field(lqcd_lattice* L, double* out, size_t n) { return clover_fields(L, 0, out, n); }
 
+int lqcd_plaquette_field(lqcd_lattice* L, int which, int dir_i, int positive_i, int dir_j, int positive_j,
+                         double* out, size_t n)
+{
+    CHECK_ARG(L && out, "null argument");
+    CHECK_ARG(which == 0 || which == 1, "which must be 0 (sij) or 1 (pij)");
+    CHECK_ARG(dir_i >= 0 && dir_i < L->g.dim && dir_j >= 0 && dir_j < L->g.dim, "direction out of range");
+    const size_t want = (size_t)L->g.nsite * 18;
+    if (n != want)
+        return fail(LQCD_ERR_INCOMPATIBLE_SIZE, "size of lattice and data are incompatible: expected " +
+                                                    std::to_string(want) + " doubles, got " + std::to_string(n));
+    DeviceGuard dg(L->ctx->device);
+    DevBuf buf;
+    HIP_TRY(hipMalloc(&buf.p, sizeof(double) * want));
+    HIP_TRY(launch_plaquette_field(L, which, dir_i, positive_i != 0, dir_j, positive_j != 0, (double*)buf.p));
+    HIP_TRY(hipMemcpyAsync(out, buf.p, sizeof(double) * want, hipMemcpyDeviceToHost, L->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    return LQCD_OK;
+}
+
 int lqcd_field_strength_device(lqcd_lattice* L, int what, double* d_out, size_t n)
 {
     CHECK_ARG(L && d_out, "null argument");

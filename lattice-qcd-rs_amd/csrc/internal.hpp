@@ -164,6 +164,9 @@ hipError_t launch_metropolis_link(const lqcd_lattice* L, void* U, double beta, d
 // clover (mode 0), f_mu_nu (1) or magnetic_field_vec (2) of U[cur] at every
 // site, written to d_out in the host layout (k_clover.hip)
 hipError_t launch_clover_fields(const lqcd_lattice* L, int mode, double* d_out);
+// sij (which 0) or pij (which 1) of U[cur] at every site for the signed
+// directions (a, pa), (b, pb), host layout (k_clover.hip)
+hipError_t launch_plaquette_field(const lqcd_lattice* L, int which, int a, bool pa, int b, bool pb, double* d_out);
 
 // ---- Gauss law (k_gauss.hip); links are U[cur], G goes to gauss_g ----
 // G(x) of the double E copy Ed, and div = sum_x |Tr(sum_a T^a G(x))| into d_div

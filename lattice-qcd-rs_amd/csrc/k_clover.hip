@@ -261,6 +261,39 @@ __global__ __launch_bounds__(kBlock) void k_clover_fields(const vec2_t<S>* __res
     }
 }
 
+// LinkMatrix::sij (WHICH 0) or pij (WHICH 1) (field.rs:743-771) at every site
+// for the signed directions (a, pa), (b, pb), in the host layout
+template <int D, typename S, int WHICH>
+__global__ __launch_bounds__(kBlock) void k_plaquette_field(const vec2_t<S>* __restrict__ U, Geom g, int a,
+                                                            bool pa, int b, bool pb, double* __restrict__ out)
+{
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= g.nsite) return;
+    const int64_t xb = site_step(g, s, b, pb), xa = site_step(g, s, a, pa);
+    mat3<double> t, sab;
+    mul<false, false>(link_signed<S>(U, g, s, b, pb), link_signed<S>(U, g, xb, a, pa), t);
+    mul<false, true>(t, link_signed<S>(U, g, xa, b, pb), sab);
+    if (WHICH == 0) {
+        store_host_matrix(out + 18 * s, sab);
+    } else {
+        mat3<double> p;
+        mul<false, true>(link_signed<S>(U, g, s, a, pa), sab, p);
+        store_host_matrix(out + 18 * s, p);
+    }
+}
+
+hipError_t launch_plaquette_field(const lqcd_lattice* L, int which, int a, bool pa, int b, bool pb, double* d_out)
+{
+    const unsigned grid = grid_for(L->g.nsite);
+    if (which == 0)
+        LQCD_DISPATCH_DS(L, k_plaquette_field<D, S, 0><<<grid, kBlock, 0, L->ctx->stream>>>(
+                                (const vec2_t<S>*)L->U[L->cur], L->g, a, pa, b, pb, d_out));
+    else
+        LQCD_DISPATCH_DS(L, k_plaquette_field<D, S, 1><<<grid, kBlock, 0, L->ctx->stream>>>(
+                                (const vec2_t<S>*)L->U[L->cur], L->g, a, pa, b, pb, d_out));
+    return hipGetLastError();
+}
+
 hipError_t launch_clover_fields(const lqcd_lattice* L, int mode, double* d_out)
 {
     const unsigned grid = grid_for(L->g.nsite);

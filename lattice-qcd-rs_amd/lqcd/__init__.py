@@ -481,6 +481,29 @@ class LatticeStateDefault:
         """``magnetic_field_vec(x)`` for every site, ``(nsite, D, 18)``."""
         return self._fields(F.lib().lqcd_magnetic_field_vec, False)
 
+    def sij_field(self, dir_i: "Direction", dir_j: "Direction") -> np.ndarray:
+        """``LinkMatrix::sij`` (src/field.rs:743-758) at every site, ``(nsite, 18)``."""
+        return self._plaquette_field(0, dir_i, dir_j)
+
+    def pij_field(self, dir_i: "Direction", dir_j: "Direction") -> np.ndarray:
+        """``LinkMatrix::pij`` (src/field.rs:761-771) at every site, ``(nsite, 18)``."""
+        return self._plaquette_field(1, dir_i, dir_j)
+
+    def sij(self, point, dir_i: "Direction", dir_j: "Direction") -> np.ndarray:
+        """``LinkMatrix::sij`` at one point (one element of ``sij_field``)."""
+        return self.sij_field(dir_i, dir_j)[self._site(point)]
+
+    def pij(self, point, dir_i: "Direction", dir_j: "Direction") -> np.ndarray:
+        """``LinkMatrix::pij`` at one point (one element of ``pij_field``)."""
+        return self.pij_field(dir_i, dir_j)[self._site(point)]
+
+    def _plaquette_field(self, which: int, dir_i: "Direction", dir_j: "Direction") -> np.ndarray:
+        n = self.lattice().number_of_points()
+        out = np.empty((n, 18), dtype=np.float64)
+        _raise(F.lib().lqcd_plaquette_field(self.handle, which, dir_i.index(), int(dir_i.is_positive()),
+                                            dir_j.index(), int(dir_j.is_positive()), F.ptr(out), out.size))
+        return out
+
     def _fields(self, fn, pairs: bool) -> np.ndarray:
         lat = self.lattice()
         d = lat.dim_st()

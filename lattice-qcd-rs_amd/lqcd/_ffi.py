@@ -145,6 +145,7 @@ SIGNATURES = {
     "lqcd_overrelaxation_sweep": (_I, [_P]),
     "lqcd_clover_field": (_I, [_P, _P, _S]),
     "lqcd_f_mu_nu_field": (_I, [_P, _P, _S]),
+    "lqcd_plaquette_field": (_I, [_P, _I, _I, _I, _I, _I, _P, _S]),
     "lqcd_magnetic_field_vec": (_I, [_P, _P, _S]),
     "lqcd_field_strength_device": (_I, [_P, _I, _P, _S]),
     "lqcd_stats_mean": (_I, [_P, _P, _S, _I, _P]),

@@ -118,3 +118,30 @@ def test_device_resident_entry_matches_host_entry():
         assert np.array_equal(got, want)
     with pytest.raises(lqcd.LqcdError):
         lqcd._raise(F.lib().lqcd_field_strength_device(st.handle, 7, buf.ptr, want.size))
+
+
+@pytest.mark.parametrize("precision", [lqcd.F64, lqcd.F32])
+def test_sij_pij_fields(precision):
+    # LinkMatrix::sij / pij (src/field.rs:743-771) at every site for every pair
+    # of signed directions, against the oracle on the stored links
+    announce(f"sij / pij fields D=3 precision={precision}")
+    ol = O.OracleLattice(3, [4, 6, 4])
+    u = ol.hot_start(23)
+    st = lqcd.LatticeStateDefault(lqcd.LatticeCyclic(1.0, [4, 6, 4], 3), 6.0, u, precision)
+    us = st.link_matrix()   # the stored links (rounded for f32)
+    for di in range(3):
+        for dj in range(3):
+            for pi in (True, False):
+                for pj in (True, False):
+                    a, b = lqcd.Direction(di, pi), lqcd.Direction(dj, pj)
+                    sf, pf = st.sij_field(a, b), st.pij_field(a, b)
+                    for s in (0, 17, 95):
+                        assert np.max(np.abs(sf[s] - ol.sij(us, s, di, pi, dj, pj))) <= 1e-12
+                        assert np.max(np.abs(pf[s] - ol.pij(us, s, di, pi, dj, pj))) <= 1e-12
+    # one point, and Tr P summed over i < j is the plaquette sum
+    p0 = st.pij((1, 2, 3), lqcd.Direction(0, True), lqcd.Direction(2, True))
+    assert np.max(np.abs(p0 - ol.pij(us, st.lattice().index_of((1, 2, 3)), 0, True, 2, True))) <= 1e-12
+    tr = sum(O.to_np(st.pij_field(lqcd.Direction(i, True), lqcd.Direction(j, True))).trace(axis1=-2, axis2=-1).sum()
+             for i in range(3) for j in range(i + 1, 3))
+    avg = st.average_trace_plaquette()
+    assert abs(tr / (st.lattice().number_of_points() * 3) - avg) <= 1e-12
