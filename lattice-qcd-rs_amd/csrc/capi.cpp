@@ -306,6 +306,9 @@ void prof_begin(lqcd_ctx* ctx, int kind, lqcd_prof_event* ev)
         }
     }
     e.kind = kind;
+    // a pooled pair keeps the count of its last use (a graph replay holds n
+    // launches); every new record starts at one launch
+    e.launches = 1;
     (void)hipEventRecord(e.start, ctx->stream);
     *ev = e;
 }

@@ -30,8 +30,12 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
+
+#include <sys/stat.h>
 
 #include "capi_util.hpp"
 #include "internal.hpp"
@@ -51,6 +55,20 @@ struct File {
 int io_fail(const std::string& what, const char* path)
 {
     return fail(LQCD_ERR_IO, what + " (" + path + ")");
+}
+
+// no C++ exception crosses the C ABI: an allocation failure becomes
+// LQCD_ERR_OUT_OF_MEMORY, anything else thrown LQCD_ERR_IO
+template <typename Fn>
+int guarded(Fn&& fn)
+{
+    try {
+        return fn();
+    } catch (const std::bad_alloc&) {
+        return fail(LQCD_ERR_OUT_OF_MEMORY, "host allocation failed while reading or writing a state");
+    } catch (const std::exception& e) {
+        return fail(LQCD_ERR_IO, std::string("state I/O: ") + e.what());
+    }
 }
 
 // ------------------------------------------------------------------ writers
@@ -188,10 +206,17 @@ void write_efield_json(Writer& w, const HostState& s)
 class Reader {
 public:
     explicit Reader(FILE* f) : f_(f) {}
+    // bytes not consumed yet of a file of `size` bytes
+    uint64_t remaining(uint64_t size) const
+    {
+        const uint64_t used = read_ - (len_ - pos_);
+        return used <= size ? size - used : 0;
+    }
     int get()
     {
         if (pos_ == len_) {
             len_ = std::fread(buf_, 1, sizeof(buf_), f_);
+            read_ += len_;
             pos_ = 0;
             if (len_ == 0) return EOF;
         }
@@ -209,6 +234,7 @@ public:
         while (n) {
             if (pos_ == len_) {
                 len_ = std::fread(buf_, 1, sizeof(buf_), f_);
+                read_ += len_;
                 pos_ = 0;
                 if (len_ == 0) return false;
             }
@@ -338,6 +364,7 @@ private:
     FILE* f_;
     char buf_[1 << 16];
     size_t pos_ = 0, len_ = 0;
+    uint64_t read_ = 0;  // bytes read from the file into buf_ so far
 };
 
 // iterate the members of a JSON object: fn(key) parses the value
@@ -462,13 +489,21 @@ int read_file(const char* path, int format, int with_e, int dim, HostState& s)
     File fh;
     fh.f = std::fopen(path, "rb");
     if (!fh.f) return io_fail(std::string("cannot open for reading: ") + std::strerror(errno), path);
+    // the file size bounds every length prefix before anything is allocated
+    uint64_t fsize = 0;
+    {
+        struct stat sb;
+        if (fstat(fileno(fh.f), &sb) != 0) return io_fail(std::string("cannot stat: ") + std::strerror(errno), path);
+        fsize = (uint64_t)sb.st_size;
+    }
     Reader r(fh.f);
     s.dim = dim;
     if (format == LQCD_FORMAT_BINCODE) {
         uint64_t n = 0;
         if (with_e) {
             if (!r.u64(n)) return io_fail("truncated E field length", path);
-            if (n > (1ull << 40)) return io_fail("implausible E field length", path);
+            // n sites of dim Su3Adjoint (8 doubles each)
+            if (n > r.remaining(fsize) / ((uint64_t)dim * 64)) return io_fail("E field length exceeds the file", path);
             s.efield.resize(n * (size_t)dim * 8);
             if (!r.bytes(s.efield.data(), s.efield.size() * sizeof(double)))
                 return io_fail("truncated E field", path);
@@ -476,7 +511,8 @@ int read_file(const char* path, int format, int with_e, int dim, HostState& s)
         }
         if (!r.f64(s.size) || !r.u64(s.points) || !r.f64(s.beta) || !r.u64(n))
             return io_fail("truncated state header", path);
-        if (n > (1ull << 40)) return io_fail("implausible link count", path);
+        // n matrices of 18 doubles
+        if (n > r.remaining(fsize) / 144) return io_fail("link count exceeds the file", path);
         s.links.resize(n * 18);
         if (!r.bytes(s.links.data(), s.links.size() * sizeof(double))) return io_fail("truncated links", path);
         if (!r.at_end()) return io_fail("trailing bytes after the state", path);
@@ -579,12 +615,12 @@ extern "C" {
 
 int lqcd_state_write(lqcd_lattice* L, double beta, const char* path, int format)
 {
-    return write_file(L, beta, 0, 0, path, format);
+    return guarded([&] { return write_file(L, beta, 0, 0, path, format); });
 }
 
 int lqcd_state_ef_write(lqcd_lattice* L, double beta, uint64_t t, const char* path, int format)
 {
-    return write_file(L, beta, 1, t, path, format);
+    return guarded([&] { return write_file(L, beta, 1, t, path, format); });
 }
 
 int lqcd_state_read_header(const char* path, int format, int with_efield, int dim, double* size,
@@ -593,7 +629,7 @@ int lqcd_state_read_header(const char* path, int format, int with_efield, int di
     CHECK_ARG(path && size && points && beta, "null argument");
     CHECK_ARG(dim >= 1 && dim <= 8, "dim must be in 1..8");
     HostState s;
-    if (int rc = read_file(path, format, with_efield, dim, s)) return rc;
+    if (int rc = guarded([&] { return read_file(path, format, with_efield, dim, s); })) return rc;
     *size = s.size;
     *points = (size_t)s.points;
     *beta = s.beta;
@@ -607,7 +643,7 @@ int lqcd_state_read_host(const char* path, int format, int with_efield, int dim,
     CHECK_ARG(path && links && (efield || !with_efield), "null argument");
     CHECK_ARG(dim >= 1 && dim <= 8, "dim must be in 1..8");
     HostState s;
-    if (int rc = read_file(path, format, with_efield, dim, s)) return rc;
+    if (int rc = guarded([&] { return read_file(path, format, with_efield, dim, s); })) return rc;
     if (s.links.size() != n_links * 18)
         return fail(LQCD_ERR_INCOMPATIBLE_SIZE, "expected " + std::to_string(s.links.size() / 18) + " links, got " +
                                                     std::to_string(n_links));
@@ -621,7 +657,7 @@ int lqcd_state_read(lqcd_lattice* L, const char* path, int format, int with_efie
 {
     CHECK_ARG(L && path, "null argument");
     HostState s;
-    if (int rc = read_file(path, format, with_efield, L->g.dim, s)) return rc;
+    if (int rc = guarded([&] { return read_file(path, format, with_efield, L->g.dim, s); })) return rc;
     for (int d = 0; d < L->g.dim; ++d)
         if ((uint64_t)L->g.extent[d] != s.points)
             return fail(LQCD_ERR_INCOMPATIBLE_SIZE, "the state's lattice does not match the handle's extents");

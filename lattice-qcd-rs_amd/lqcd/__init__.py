@@ -930,15 +930,22 @@ class HybridMonteCarloDiagnostic:
     (src/simulation/state.rs:1093-1108; ``project_gauss=False`` skips it),
     integrates ``number_of_steps`` steps of ``delta_t`` and accepts with
     p = min(1, exp(H_old - H_new)).
+
+    The reference's projection loop (``project_to_gauss``, field.rs:1265-1294)
+    has no bound; here it stops with ``StateInitializationError`` after
+    ``gauss_max_iter`` checks.  Its linear convergence slows with the volume
+    (25-56 checks at 8^4, about 2,500 at 32^4), so the default bound,
+    ``gauss_max_iter=None``, grows with the lattice: 1000 + sites / 256
+    (5,096 at 32^4, 42,472 at 48^3 x 96).
     """
 
     def __init__(self, delta_t: float, number_of_steps: int, integrator: SymplecticEulerGpu,
-                 seed: int, project_gauss: bool = True, gauss_max_iter: int = 1000):
+                 seed: int, project_gauss: bool = True, gauss_max_iter: Optional[int] = None):
         self._dt = float(delta_t)
         self._n = int(number_of_steps)
         self._integrator = integrator
         self._project = bool(project_gauss)
-        self._gauss_max_iter = int(gauss_max_iter)
+        self._gauss_max_iter = None if gauss_max_iter is None else int(gauss_max_iter)
         self._seed = int(seed) & 0xFFFFFFFFFFFFFFFF
         self._rng_state = self._seed
         self._counter = 0
@@ -948,6 +955,13 @@ class HybridMonteCarloDiagnostic:
 
     def delta_t(self) -> float:
         return self._dt
+
+    def gauss_max_iter(self, number_of_points: int) -> int:
+        """The bound on the Gauss-projection checks for a lattice of
+        ``number_of_points`` sites."""
+        if self._gauss_max_iter is not None:
+            return self._gauss_max_iter
+        return 1000 + int(number_of_points) // 256
 
     def number_of_steps(self) -> int:
         return self._n
@@ -983,7 +997,8 @@ class HybridMonteCarloDiagnostic:
         _raise(F.lib().lqcd_efield_random(state.handle, self._seed, self._counter, sigma))
         self._counter += 1
         if self._project:
-            _raise(F.lib().lqcd_efield_project_gauss(state.handle, self._gauss_max_iter, None))
+            _raise(F.lib().lqcd_efield_project_gauss(
+                state.handle, self.gauss_max_iter(state.lattice().number_of_points()), None))
         self.trajectory(state, self._next_u64())
         return state
 

@@ -78,3 +78,25 @@ def test_graph_profiling_counts_launches():
     k, _ = ctx.profile_read(F.KERNEL_KICK)
     ctx.profile_enable(0)
     assert n == 18 and k == 3 and ms > 0.0
+
+
+def test_profiling_sessions_reuse_events_with_fresh_counts():
+    # a session whose replays count n launches per event pair, then a session
+    # of eager kicks only: the pooled event pairs must count one launch each
+    announce("profiling: second session after graph replays")
+    _, b = pair(lqcd.F32, seed=5)
+    ef = lqcd.LatticeStateEFSyncDefault.new_random_e_state(b, 7, 0, project=False)
+    integ = lqcd.SymplecticEulerGpu(lqcd.DRIFT_EXP, lqcd.SCHEME_LEAPFROG)
+    ctx = b._dev.ctx
+    ctx.profile_enable(1 << F.KERNEL_KICK_DRIFT)
+    for _ in range(4):
+        ef.simulate_symplectic_n(integ, 0.02, 10)
+    n, _ = ctx.profile_read(F.KERNEL_KICK_DRIFT)
+    assert n == 40
+    ctx.profile_enable((1 << F.KERNEL_KICK_DRIFT) | (1 << F.KERNEL_KICK))
+    for _ in range(5):
+        integ.integrate_efield(ef, 0.01)
+    k, ms = ctx.profile_read(F.KERNEL_KICK)
+    n, _ = ctx.profile_read(F.KERNEL_KICK_DRIFT)
+    ctx.profile_enable(0)
+    assert (k, n) == (5, 0) and ms > 0.0

@@ -114,3 +114,20 @@ def test_malformed_files_are_refused(tmp_path):
     assert header(p, 7, 0, dim)[0] == F.ERR_INVALID_ARGUMENT
     rc, _, _ = read_host(p, F.FORMAT_BINCODE, 0, dim, nlink + 1)
     assert rc == F.ERR_INCOMPATIBLE_SIZE
+
+
+def test_oversized_length_prefixes_are_refused(tmp_path):
+    # a length prefix larger than the file: an I/O error before any
+    # allocation, not an exception through the C ABI
+    dim, points = 3, 2
+    links, e, nlink = random_state(dim, points, 5)
+    p = tmp_path / "big"
+    for n in (2 ** 40, 2 ** 62, 2 ** 64 - 1, nlink + 1):
+        bad = state_bincode(1.0, points, 6.0, links)
+        # the link count is the u64 after size (f64), points (u64) and beta (f64)
+        bad = bad[:24] + struct.pack("<Q", n) + bad[32:]
+        p.write_bytes(bad)
+        assert header(p, F.FORMAT_BINCODE, 0, dim)[0] == F.ERR_IO, n
+        good_ef = ef_bincode(1.0, points, 6.0, links, e, 3, dim)
+        p.write_bytes(struct.pack("<Q", n) + good_ef[8:])
+        assert header(p, F.FORMAT_BINCODE, 1, dim)[0] == F.ERR_IO, n
