@@ -15,8 +15,8 @@ kernel over its average duration from HIP events recorded around every launch
 in the timed region.  The algorithmic bytes of one leapfrog step are those of
 one force evaluation (read U and E, write E: 34 reals per link) plus one link
 drift (read U and E, write U: 44 reals), 78 reals per link; the fused kernel
-does both in one pass, which needs 52 reals per link (46 when U is read as
-its first two rows), reported beside it.
+does both in one pass, which needs 52 reals per link (40 when U is read and
+written as its first two rows), reported beside it.
 mixed_precision_errors_vs_oracle = one trajectory of the workload from the
 oracle's f64 hot start and momenta, in the FP32-mixed and FP64 device legs and
 in the oracle (f64): per-observable differences.  The oracle's trajectory is
@@ -39,6 +39,8 @@ Other workloads (--workload), one JSON line each, same keys:
            four OverrelaxationSweepRotation per step (checkerboard, device streams).
   stats    statistics::mean_and_variance (two-pass) of 2^27 f64 values resident
            in device memory.
+  variants every MD work decomposition (lqcd_md_variant_set 0..4) on the
+           headline lattice: per-launch time of the fused kick+drift kernel.
   small    8^4 FP64: plaquette, HMC trajectories of 20 steps dt = 0.01
            (reference kick-drift-kick scheme, additive drift) and Metropolis
            sweeps (spread 0.1) with reunitarisation every 10 sweeps.
@@ -65,15 +67,18 @@ REALS_FORCE = 34
 REALS_DRIFT = 44
 REALS_STEP = REALS_FORCE + REALS_DRIFT   # one leapfrog step: a force and a drift
 # what the fused kick+drift kernel itself moves in one pass: read U and E,
-# write E and U; 46 when U is read as its first two rows (links in SU(3))
+# write E and U; 40 when the links are in SU(3) and U is read and written as
+# its first two rows (12 reals each way)
 REALS_KICK_DRIFT = 52
-REALS_KICK_DRIFT_R12 = 46
+REALS_KICK_DRIFT_R12 = 40
 REALS_KICK = REALS_FORCE
 REALS_KICK_R12 = 28
-# VALU instructions per site of the fused f32 kernel (k_md_site_xcd<4, float, float, 1, R12>),
-# counted in its gfx950 assembly (hipcc -O3 --cuda-device-only -S), and the issue rate of
+# VALU instructions per site of the fused f32 exp-drift kernels by (MD variant,
+# two-row links), counted in their gfx950 assembly (hipcc -O3 --cuda-device-only -S
+# of k_md.hip: k_md_site_xcd<4, float, float, 1, R12> for variants 2 and 3,
+# k_md_site_tile<4, float, float, 1> for variant 4), and the issue rate of
 # the vector ALU: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per 4 cycles per SIMD
-VALU_PER_SITE = {True: 7171, False: 6902}
+VALU_PER_SITE = {(2, True): 7031, (3, True): 7031, (4, True): 7667, (2, False): 6902, (3, False): 6902}
 VALU_WAVE_INSTR_PER_S = 256 * 4 * 2.4e9 / 4
 REALS_PLAQ = 18
 REALS_EENERGY = 8
@@ -99,7 +104,7 @@ def parse():
                    help="only run the 1 GiB streaming copy (byte-counter calibration)")
     p.add_argument("--workload", default="hmc32",
                    choices=["hmc32", "adjoint", "batched", "large", "mh3d", "small", "fields", "state", "stats",
-                            "sweeps"])
+                            "sweeps", "variants"])
     p.add_argument("--oracle-trajectory", type=int, default=1,
                    help="1: the per-observable error table against the oracle's f64 trajectory, "
                         "which is also the CPU baseline (0 = skip)")
@@ -233,7 +238,7 @@ def oracle_error_table(lqcd, args):
     t_cpu = time.perf_counter() - t0
     ref = {"h_old": h_old, "h_new": h_new, "dh": h_new - h_old, "h_e": h_e,
            "plaquette": ol.average_trace_plaquette(u_new).real / 3.0}
-    ref_acc = O.bernoulli_accept(O.clamp_probability(math.exp(h_old - h_new)), uniform)
+    ref_acc = O.bernoulli_accept(O.replacement_probability(h_old, h_new), uniform)
     legs = {}
     integ = lqcd.SymplecticEulerGpu(lqcd.DRIFT_EXP, lqcd.SCHEME_LEAPFROG)
     for name, prec in (("f32_mixed", lqcd.F32), ("f64", lqcd.F64)):
@@ -754,6 +759,62 @@ def bench_sweeps(args, lqcd):
         sweeps_per_s=round(5 * args.steps / seconds, 3))
 
 
+def bench_variants(args, lqcd):
+    """Every MD work decomposition (lqcd_md_variant_set 0..4) timed on the
+    headline lattice: the 40 fused kick+drift launches of one leapfrog
+    integration (f32, exp drift, links read and written as two rows),
+    replayed from the captured graph, per-launch time from HIP events; two
+    alternated rounds, the better kept per variant.  The links are those the
+    headline's timed trajectories start from: the hot start after --warmup
+    HMC trajectories of the headline workload.  The line's value is the link
+    updates/s of the fastest decomposition."""
+    F = lqcd._ffi
+    L = args.extent
+    st = lqcd.LatticeStateDefault.new_determinist(1.0, args.beta, L, args.seed, 4, lqcd.F32)
+    integ = lqcd.SymplecticEulerGpu(lqcd.DRIFT_EXP, lqcd.SCHEME_LEAPFROG)
+    hmc = lqcd.HybridMonteCarloDiagnostic(args.dt, args.md_steps, integ, args.seed, project_gauss=False)
+    for _ in range(args.warmup):
+        hmc.next_element(st)
+        st.normalize_link_matrices()
+    ef = lqcd.LatticeStateEFSyncDefault(st)
+    ctx = st._dev.ctx
+    nlink = st.lattice().number_of_canonical_links_space()
+    u0 = st.link_matrix()
+    names = {0: "k_md_link (2-D grid)", 1: "k_md_link_xcd", 2: "k_md_site_xcd", 3: "k_md_site_xcd, slab order",
+             4: "k_md_site_tile (LDS tile of x_0/x_1 rows)"}
+    best = {}
+    for _ in range(2):
+        for v in range(5):
+            st.set_link_matrix(u0)
+            ef.reset_e_field(args.seed, 0, 0.5 / args.beta)
+            st.set_md_variant(v)
+            ef.simulate_symplectic_n(integ, args.dt, 2)   # capture and warm up
+            ef.reset_e_field(args.seed, 0, 0.5 / args.beta)
+            ctx.profile_enable(1 << F.KERNEL_KICK_DRIFT)
+            ctx.synchronize()
+            ef.simulate_symplectic_n(integ, args.dt, args.md_steps)
+            n, ms = ctx.profile_read(F.KERNEL_KICK_DRIFT)
+            ctx.profile_enable(0)
+            us = ms / n * 1e3
+            if v not in best or us < best[v]["avg_us"]:
+                best[v] = {"avg_us": round(us, 2), "launches": n, "kernel": names[v],
+                           "GB_s_alg": round(nlink * REALS_STEP * 4 / (us * 1e-6) / 1e9, 1),
+                           "frac_alg": round(nlink * REALS_STEP * 4 / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+                           "two_row_reads": st.md_reconstruct_last()}
+    fastest = min(best, key=lambda v: best[v]["avg_us"])
+    us = best[fastest]["avg_us"]
+    rate = {"launches": best[fastest]["launches"], "avg_us": us}
+    cfg = {"workload": (f"{L}^4 D=4 FP32 links, exp-drift leapfrog, beta {args.beta}, hot start after "
+                        f"{args.warmup} HMC trajectories: {args.md_steps} fused kick+drift launches per MD "
+                        f"work decomposition"),
+           "lattice": [L] * 4, "links": nlink}
+    roof = md_roofline(rate, nlink, 4, best[fastest]["two_row_reads"], names[fastest])
+    return contract_line(args, "MD kick+drift link updates/s by work decomposition, 32^4 FP32",
+                         nlink / (us * 1e-6), "link updates/s", args.md_steps, us * 1e-3 * args.md_steps / 1e3,
+                         "f32", cfg, roof, None, variants={str(v): best[v] for v in sorted(best)},
+                         fastest_variant=fastest)
+
+
 def bench_stats(args, lqcd):
     """statistics::mean_and_variance (statistics/mod.rs:375-399, two passes:
     the sum, then sum (x - mean)^2) of 2^27 f64 values resident in device
@@ -920,7 +981,8 @@ def main():
     if args.workload != "hmc32":
         run = {"adjoint": bench_adjoint, "batched": bench_batched, "large": bench_large, "mh3d": bench_mh3d,
                "small": bench_small, "fields": bench_fields,
-               "state": bench_state, "stats": bench_stats, "sweeps": bench_sweeps}[args.workload]
+               "state": bench_state, "stats": bench_stats, "sweeps": bench_sweeps,
+               "variants": bench_variants}[args.workload]
         print(json.dumps(run(args, lqcd)))
         return
 
@@ -951,7 +1013,8 @@ def main():
     variant = mixed["md_variant"]
     n_kd, ms_kd = mixed["kernels"][F.KERNEL_KICK_DRIFT]
     rate_kd = {"launches": n_kd, "avg_us": ms_kd / n_kd * 1e3}
-    kname = (f"k_md_{'site' if variant >= 2 else 'link'}_xcd<4,float,float,EXP> "
+    kbase = {0: "k_md_link", 1: "k_md_link_xcd", 2: "k_md_site_xcd", 3: "k_md_site_xcd", 4: "k_md_site_tile"}
+    kname = (f"{kbase.get(variant, 'k_md')}<4,float,float,EXP> "
              f"(fused force + momentum kick + exp link drift, MD variant {variant}"
              f"{', links read as two rows' if r12 else ''})")
     roofline = md_roofline(rate_kd, nlink, 4, r12, kname)
@@ -961,10 +1024,11 @@ def main():
         roofline["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of the same kernel build, "
                                       "scaled by the copy-kernel calibration (profiles/kick_drift_32x4_f32_pmc.md)")
     roofline["avg_us"] = round(rate_kd["avg_us"], 2)
-    if variant >= 2:
+    if (variant, r12) in VALU_PER_SITE:
         # the second roof of this kernel: vector ALU issue (no MFMA, no packed FP32)
-        floor_s = (nlink // 4) / 64 * VALU_PER_SITE[r12] / VALU_WAVE_INSTR_PER_S
-        roofline["valu"] = {"instr_per_site": VALU_PER_SITE[r12], "floor_us": round(floor_s * 1e6, 1),
+        valu = VALU_PER_SITE[(variant, r12)]
+        floor_s = (nlink // 4) / 64 * valu / VALU_WAVE_INSTR_PER_S
+        roofline["valu"] = {"instr_per_site": valu, "floor_us": round(floor_s * 1e6, 1),
                             "frac": round(floor_s * 1e6 / rate_kd["avg_us"], 4)}
     kernels = {}
     for leg, tag, elem in ((mixed, "f32_", 4), (f64, "f64_", 8)):

@@ -101,6 +101,8 @@ hipError_t launch_batch_sums(const lqcd_batch* b, const void* U, const void* E, 
 hipError_t launch_batch_accept(const lqcd_batch* b);
 hipError_t launch_batch_accept_copy(const lqcd_batch* b, const void* U_final);
 hipError_t launch_batch_reunitarize(const lqcd_batch* b);
+// row 2 of every chain's links in U rebuilt from rows 0, 1 (SU(3) links)
+hipError_t launch_batch_complete_row2(const lqcd_batch* b, void* U);
 hipError_t launch_batch_counter_advance(const lqcd_batch* b);
 
 } // namespace lqcd

@@ -230,7 +230,8 @@ static int buffer_after(const lqcd_lattice* L, size_t n)
 // n steps of the chosen scheme starting from U[cur]; *last gets the buffer
 // holding the final links.  U[cur] is left untouched.  An integration with
 // the exp drift keeps the links in SU(3) (up to rounding), so when the
-// starting links are in SU(3) its kernels read only their first two rows.
+// starting links are in SU(3) its kernels read only their first two rows and
+// store only the first two rows of the links they drift.
 int integrate_steps(lqcd_lattice* L, double dt, size_t n, int drift, int scheme, int* last)
 {
     R12Scope scope{L};
@@ -272,6 +273,9 @@ int integrate_steps(lqcd_lattice* L, double dt, size_t n, int drift, int scheme,
         }
         HIP_TRY(launch_kick(L, L->U[src], L->E, dt / 2.0));
     }
+    // two-row kernels store the drifted links as rows 0, 1: the final links
+    // get their row 2 back before anything else reads them
+    if (L->md_r12 && n > 0) HIP_TRY(launch_complete_row2(L, L->U[src]));
     *last = src;
     return LQCD_OK;
 }
@@ -464,7 +468,7 @@ int lqcd_lattice_destroy(lqcd_lattice* L)
     if (L->mh_p) (void)hipFree(L->mh_p);
     if (L->mh_acc) (void)hipFree(L->mh_acc);
     if (L->gauss_g) (void)hipFree(L->gauss_g);
-    for (int k = 0; k < 2; ++k)
+    for (int k = 0; k < 3; ++k)
         if (L->gauss_e[k]) (void)hipFree(L->gauss_e[k]);
     for (auto& ge : L->md_graph)
         if (ge) (void)hipGraphExecDestroy(ge);
@@ -580,7 +584,7 @@ static int ensure_gauss_buffers(lqcd_lattice* L)
 {
     const size_t npad = (size_t)L->g.npad;
     if (!L->gauss_g) HIP_TRY(hipMalloc(&L->gauss_g, sizeof(double) * 2 * 9 * npad));
-    for (int k = 0; k < 2; ++k)
+    for (int k = 0; k < 3; ++k)
         if (!L->gauss_e[k]) HIP_TRY(hipMalloc((void**)&L->gauss_e[k], sizeof(double) * 8 * npad * L->g.dim));
     return LQCD_OK;
 }
@@ -679,23 +683,38 @@ int lqcd_efield_project_gauss(lqcd_lattice* L, size_t max_iter, size_t* iteratio
     CHECK_ARG(L, "lattice is null");
     DeviceGuard dg(L->ctx->device);
     if (int rc = ensure_gauss_buffers(L)) return rc;
-    double* cur = L->gauss_e[0];
-    double* nxt = L->gauss_e[1];
     // project_to_gauss, src/field.rs:1265-1294: one step, then while the
-    // deviation is above N * 4 * 8 * 10 * eps, four more steps
-    HIP_TRY(launch_efield_to_double(L, L->E, cur));
-    HIP_TRY(launch_gauss(L, cur, L->result));
-    HIP_TRY(launch_gauss_step(L, cur, nxt));
-    std::swap(cur, nxt);
+    // deviation is above N * 4 * 8 * 10 * eps, four more steps.  Each step is
+    // one k_gauss_fused launch, which also sums the deviation of its input: a
+    // check is the first launch of its group of four.  The host reads that
+    // deviation while the other three steps run; they write the two buffers
+    // other than the checked field, which stays the result if the check passes.
+    double* const buf[3] = {L->gauss_e[0], L->gauss_e[1], L->gauss_e[2]};
+    int cur = 1;
+    HIP_TRY(launch_efield_to_double(L, L->E, buf[0]));
+    HIP_TRY(launch_gauss_fused(L, buf[0], buf[1], L->result + 1));
     const double threshold = 2.220446049250313e-16 * (double)(L->g.nsite * 4 * 8 * 10);
+    hipEvent_t checked;
+    HIP_TRY(hipEventCreateWithFlags(&checked, hipEventDisableTiming));
+    struct EventGuard {
+        hipEvent_t e;
+        ~EventGuard() { (void)hipEventDestroy(e); }
+    } eg{checked};
     size_t it = 0;
     bool converged = false;
     double div = 0.0;
     for (; it < max_iter; ++it) {
-        HIP_TRY(launch_gauss(L, cur, L->result));
-        if (int rc = read_results(L, 1)) return rc;
+        const int n1 = (cur + 1) % 3, n2 = (cur + 2) % 3;
+        HIP_TRY(launch_gauss_fused(L, buf[cur], buf[n1], L->result));
+        HIP_TRY(hipMemcpyAsync(L->host_result, L->result, sizeof(double), hipMemcpyDeviceToHost, L->ctx->stream));
+        HIP_TRY(hipEventRecord(checked, L->ctx->stream));
+        HIP_TRY(launch_gauss_fused(L, buf[n1], buf[n2], L->result + 1));
+        HIP_TRY(launch_gauss_fused(L, buf[n2], buf[n1], L->result + 1));
+        HIP_TRY(launch_gauss_fused(L, buf[n1], buf[n2], L->result + 1));
+        HIP_TRY(hipEventSynchronize(checked));
         div = L->host_result[0];
         if (std::isnan(div)) {
+            HIP_TRY(hipStreamSynchronize(L->ctx->stream));
             if (iterations) *iterations = it;
             return fail(LQCD_ERR_GAUSS_PROJECTION, "gauss projection could not finish: deviation is NaN");
         }
@@ -703,21 +722,18 @@ int lqcd_efield_project_gauss(lqcd_lattice* L, size_t max_iter, size_t* iteratio
             converged = true;
             break;
         }
-        for (int k = 0; k < 4; ++k) {
-            if (k > 0) HIP_TRY(launch_gauss(L, cur, L->result + 1));
-            HIP_TRY(launch_gauss_step(L, cur, nxt));
-            std::swap(cur, nxt);
-        }
+        cur = n2;
     }
     if (iterations) *iterations = it;
     if (!converged) {
+        HIP_TRY(hipStreamSynchronize(L->ctx->stream));
         char msg[160];
         std::snprintf(msg, sizeof(msg),
                       "gauss projection could not finish within %zu iterations (deviation %.6e, threshold %.6e)",
                       max_iter, div, threshold);
         return fail(LQCD_ERR_GAUSS_PROJECTION, msg);
     }
-    HIP_TRY(launch_efield_from_double(L, cur, L->E));
+    HIP_TRY(launch_efield_from_double(L, buf[cur], L->E));
     HIP_TRY(hipStreamSynchronize(L->ctx->stream));
     return LQCD_OK;
 }

@@ -95,6 +95,8 @@ hipError_t batch_trajectory(const lqcd_batch* b, double dt, size_t n, int drift,
     hipError_t e = launch_batch_momenta(b);
     if (e == hipSuccess) e = launch_batch_sums(b, b->U[0], b->E, b->d_h_old);
     if (e == hipSuccess) e = batch_integrate(b, b->config, dt, n, drift, scheme, &last);
+    // two-row MD kernels store rows 0, 1 of the links they drift
+    if (e == hipSuccess && b->md_r12 && n > 0) e = launch_batch_complete_row2(b, b->U[last]);
     if (e == hipSuccess) e = launch_batch_sums(b, b->U[last], b->E, b->d_h_new);
     if (e == hipSuccess) e = launch_batch_accept(b);
     if (e == hipSuccess) e = launch_batch_accept_copy(b, b->U[last]);

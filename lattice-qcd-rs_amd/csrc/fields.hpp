@@ -153,6 +153,44 @@ __device__ __forceinline__ void store_link(vec2_t<S>* __restrict__ U, int64_t np
     }
 }
 
+// Rows 0 and 1 of U_mu(x) only (planes 0-5): for links in SU(3) whose
+// readers rebuild row 2 (load_link_r12) or which get it back from
+// complete_row2_link before any other reader.
+template <typename T, typename S, int CP = kCacheDefault>
+__device__ __forceinline__ void store_link_rows01(vec2_t<S>* __restrict__ U, int64_t npad, int mu, int64_t s,
+                                                  const mat3<T>& m)
+{
+    using E = vec2_t<S>;
+    const __amdgpu_buffer_rsrc_t r = planes_rsrc(U + (int64_t)mu * 9 * npad, npad, 9);
+    const uint32_t off = (uint32_t)s * (uint32_t)sizeof(E);
+    const uint32_t plane = (uint32_t)npad * (uint32_t)sizeof(E);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        E v;
+        v.x = S(m.e[k].re);
+        v.y = S(m.e[k].im);
+        buf_store<E, CP>(r, off, (uint32_t)k * plane, v);
+    }
+}
+
+// Row 2 of U_mu(x) only (planes 6-8).
+template <typename T, typename S, int CP = kCacheDefault>
+__device__ __forceinline__ void store_link_row2(vec2_t<S>* __restrict__ U, int64_t npad, int mu, int64_t s,
+                                                const mat3<T>& m)
+{
+    using E = vec2_t<S>;
+    const __amdgpu_buffer_rsrc_t r = planes_rsrc(U + (int64_t)mu * 9 * npad, npad, 9);
+    const uint32_t off = (uint32_t)s * (uint32_t)sizeof(E);
+    const uint32_t plane = (uint32_t)npad * (uint32_t)sizeof(E);
+#pragma unroll
+    for (int k = 6; k < 9; ++k) {
+        E v;
+        v.x = S(m.e[k].re);
+        v.y = S(m.e[k].im);
+        buf_store<E, CP>(r, off, (uint32_t)k * plane, v);
+    }
+}
+
 template <typename T, typename S, int CP = kCacheDefault>
 __device__ __forceinline__ void load_e(const S* __restrict__ E, int64_t npad, int mu, int64_t s, T* e)
 {

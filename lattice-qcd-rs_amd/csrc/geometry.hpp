@@ -50,6 +50,24 @@ __device__ __forceinline__ Neighbours<D> neighbours(const Geom& g, int64_t s)
     return n;
 }
 
+// Workgroups are dispatched round robin over the 8 XCDs (workgroup b runs on
+// XCD b mod 8).  Give XCD k the contiguous logical range
+// [k q + min(k, r), ...) of n = 8 q + r blocks.
+__device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t n)
+{
+    const int64_t k = b & 7, i = b >> 3, q = n >> 3, r = n & 7;
+    return k * q + (k < r ? k : r) + i;
+}
+
+// Site block of logical block j for the slab order: P blocks per slice of the
+// slowest direction, G slices per XCD slab; within a slab the slice index runs
+// fastest.
+__device__ __forceinline__ int64_t slab_block(int64_t j, int64_t P, int64_t G)
+{
+    const int64_t q = j / (G * P), r = j % (G * P);
+    return (q * G + r % G) * P + r / G;
+}
+
 template <int D>
 __device__ __forceinline__ int site_parity(const Geom& g, int64_t s)
 {

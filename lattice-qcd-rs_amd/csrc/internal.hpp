@@ -60,7 +60,7 @@ struct lqcd_lattice {
     // established; -1 unknown; -2 known to be off SU(3)
     int64_t su3_known = -1;
     void* gauss_g = nullptr;       // Gauss law G(x): 9 planes of double2
-    double* gauss_e[2] = {nullptr, nullptr};  // double copies of E for the projection
+    double* gauss_e[3] = {nullptr, nullptr, nullptr};  // double copies of E for the projection
 };
 
 namespace lqcd {
@@ -109,6 +109,8 @@ hipError_t launch_efield_zero(const lqcd_lattice* L, void* E);
 hipError_t launch_device_momenta(const lqcd_lattice* L, void* E, uint64_t seed, uint64_t counter,
                                  double sigma);
 hipError_t launch_reunitarize(const lqcd_lattice* L, void* U);
+// row 2 of every link of U rebuilt from rows 0, 1 (links in SU(3))
+hipError_t launch_complete_row2(const lqcd_lattice* L, void* U);
 hipError_t launch_links_random(const lqcd_lattice* L, void* U, uint64_t seed);
 hipError_t launch_stream_copy(hipStream_t stream, const void* src, void* dst, size_t bytes);
 
@@ -125,7 +127,17 @@ hipError_t launch_mh_stats(const lqcd_lattice* L, double* d_out2);
 
 // ---- molecular dynamics (k_md.hip) ----
 enum DriftMode { DRIFT_ADDITIVE = 0, DRIFT_EXP = 1 };
-enum MdVariant { VAR_LINK = 0, VAR_LINK_XCD = 1, VAR_SITE_XCD = 2, VAR_SITE_SLAB = 3, VAR_COUNT = 4 };
+enum MdVariant {
+    VAR_LINK = 0,
+    VAR_LINK_XCD = 1,
+    VAR_SITE_XCD = 2,
+    VAR_SITE_SLAB = 3,
+    VAR_SITE_TILE = 4,   // falls back to VAR_SITE_SLAB where md_tile_fits is false
+    VAR_COUNT = 5
+};
+// VAR_SITE_TILE's LDS tile: whether it applies to the lattice, and its bytes
+bool md_tile_fits(const lqcd_lattice* L);
+size_t md_tile_bytes(const lqcd_lattice* L);
 hipError_t launch_kick_variant(const lqcd_lattice* L, int variant, const void* U, void* E, double kick);
 hipError_t launch_kick_drift_variant(const lqcd_lattice* L, int variant, const void* U_in, void* U_out,
                                      void* E, double kick, double dt, int drift_mode);
@@ -171,9 +183,10 @@ hipError_t launch_plaquette_field(const lqcd_lattice* L, int which, int a, bool 
 // ---- Gauss law (k_gauss.hip); links are U[cur], G goes to gauss_g ----
 // G(x) of the double E copy Ed, and div = sum_x |Tr(sum_a T^a G(x))| into d_div
 hipError_t launch_gauss(const lqcd_lattice* L, const double* Ed, double* d_div);
-// one project_to_gauss_step from Ed_in with the G of Ed_in (already in gauss_g)
-hipError_t launch_gauss_step(const lqcd_lattice* L, const double* Ed_in, double* Ed_out);
 hipError_t launch_gauss_to_aos(const lqcd_lattice* L, double* d_out);
+// one project_to_gauss_step Ed_in -> Ed_out with G computed in the kernel
+// (gauss_g untouched), and div of Ed_in into d_div
+hipError_t launch_gauss_fused(const lqcd_lattice* L, const double* Ed_in, double* Ed_out, double* d_div);
 hipError_t launch_efield_to_double(const lqcd_lattice* L, const void* E, double* Ed);
 hipError_t launch_efield_from_double(const lqcd_lattice* L, const double* Ed, void* E);
 // sum of nblocks partials (slot 0) into d_out1

@@ -50,10 +50,13 @@ __global__ __launch_bounds__(kBlock) void k_batch_md(const vec2_t<S>* __restrict
         const vec2_t<S>* ui = U_in + d.u_off;
         vec2_t<S>* uo = DR >= 0 ? U_out + d.u_off : nullptr;
         S* e = E + d.e_off;
+        // two-row kernels store the drifted links as rows 0, 1 (the
+        // trajectory rebuilds row 2 of its final links, k_batch_links_map<2>)
+        constexpr bool R01 = R12 && DR == DRIFT_EXP;
         if (SITE)
-            md_site<D, S, S, DR, R12>(ui, uo, e, d.g, s, fs, (S)kick, df);
+            md_site<D, S, S, DR, R12, false, R01>(ui, uo, e, d.g, s, fs, (S)kick, df);
         else
-            md_link<D, S, S, DR, R12>(ui, uo, e, d.g, mu, s, fs, (S)kick, df);
+            md_link<D, S, S, DR, R12, R01>(ui, uo, e, d.g, mu, s, fs, (S)kick, df);
     }
 }
 
@@ -100,7 +103,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_hot_start(vec2_t<S>* __restric
 }
 
 // MODE 0: identity links; 1: reunitarise in place (LinkMatrix::normalize,
-// src/field.rs:897-901, in double arithmetic)
+// src/field.rs:897-901, in double arithmetic); 2: rebuild row 2 of SU(3)
+// links from rows 0, 1 in the storage precision (after two-row MD kernels)
 template <int D, typename S, int MODE>
 __global__ __launch_bounds__(kBlock) void k_batch_links_map(vec2_t<S>* __restrict__ U,
                                                             const ChainDesc* __restrict__ cd,
@@ -113,6 +117,10 @@ __global__ __launch_bounds__(kBlock) void k_batch_links_map(vec2_t<S>* __restric
     if (s >= d.g.nsite) return;
 #pragma unroll
     for (int mu = 0; mu < D; ++mu) {
+        if constexpr (MODE == 2) {
+            store_link_row2<S, S>(U + d.u_off, d.g.npad, mu, s, load_link_r12<S, S>(U + d.u_off, d.g.npad, mu, s));
+            continue;
+        }
         mat3<double> m;
         if (MODE == 0) {
             set_identity(m);
@@ -399,6 +407,13 @@ hipError_t launch_batch_reunitarize(const lqcd_batch* b)
     BATCH_DISPATCH(b, k_batch_links_map<D, S, 1><<<tile_grid(b), kBlock, 0, b->ctx->stream>>>(
                           (vec2_t<S>*)b->U[0], b->d_chains, b->d_tile_chain, b->ntile));
     prof_end(b->ctx, &ev);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_complete_row2(const lqcd_batch* b, void* U)
+{
+    BATCH_DISPATCH(b, k_batch_links_map<D, S, 2><<<tile_grid(b), kBlock, 0, b->ctx->stream>>>(
+                          (vec2_t<S>*)U, b->d_chains, b->d_tile_chain, b->ntile));
     return hipGetLastError();
 }
 

@@ -126,6 +126,18 @@ __global__ __launch_bounds__(kBlock) void k_reunitarize(vec2_t<S>* __restrict__ 
     store_link<double, S>(U, g.npad, mu, s, m);
 }
 
+// Row 2 of every link rebuilt from rows 0 and 1 (conj(row0 x row1), in the
+// storage precision, the arithmetic of load_link_r12): after an integration
+// whose kernels stored only rows 0, 1 of the SU(3) links they drifted.
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_complete_row2(vec2_t<S>* __restrict__ U, Geom g)
+{
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int mu = blockIdx.y;
+    if (s >= g.nsite) return;
+    store_link_row2<S, S>(U, g.npad, mu, s, load_link_r12<S, S>(U, g.npad, mu, s));
+}
+
 // Streaming copy of n 16-byte words: the reference point for achievable HBM
 // bandwidth and for calibrating byte counters.
 __global__ __launch_bounds__(kBlock) void k_stream_copy(const uint4* __restrict__ a, uint4* __restrict__ b,
@@ -203,6 +215,12 @@ hipError_t launch_links_random(const lqcd_lattice* L, void* U, uint64_t seed)
 {
     LQCD_DISPATCH_DS(L, k_links_random<D, S><<<link_grid(L), kBlock, 0, L->ctx->stream>>>(
                             (vec2_t<S>*)U, L->g, seed));
+    return hipGetLastError();
+}
+
+hipError_t launch_complete_row2(const lqcd_lattice* L, void* U)
+{
+    LQCD_DISPATCH_S(L, k_complete_row2<S><<<link_grid(L), kBlock, 0, L->ctx->stream>>>((vec2_t<S>*)U, L->g));
     return hipGetLastError();
 }
 

@@ -9,7 +9,9 @@
 //
 // The iteration works on double-precision copies of E (for f32 storage as
 // well: the reference's stop rule, div <= N * 320 * eps, is far below what
-// an f32 field can represent) and G is kept as double planes.
+// an f32 field can represent).  EField::gauss keeps G as double planes
+// (k_gauss); the projection's steps compute it where they need it
+// (k_gauss_fused).
 #include "dispatch.hpp"
 #include "observables_device.hpp"
 
@@ -91,32 +93,74 @@ __global__ __launch_bounds__(kBlock) void k_gauss(const vec2_t<S>* __restrict__ 
     block_reduce_to<1>(acc, partials + (int64_t)blockIdx.x * kReduceSlots);
 }
 
+// One project_to_gauss_step of the whole field with G computed on the fly:
+// Ed_out = step(Ed_in), and the block partials of div(Ed_in).  The kernel
+// walks the lattice in tiles of kBlock consecutive sites (tile t = blockIdx.x
+// + k gridDim.x, the sites of k_gauss's grid-stride loop, so the div partials
+// are summed in the same order as there).  G(x) of the tile goes to LDS; the
+// step of link (x, i) takes G(x + i) from there when x + i lies in the tile
+// (x + 0 always for extent[0] dividing kBlock, x + 1 for all but the last
+// row) and otherwise computes it again from E and U (gauss_at, the same
+// arithmetic, so the value is the same either way).  G is never written to
+// HBM.
 template <int D, typename S>
-__global__ __launch_bounds__(kBlock) void k_gauss_step(const vec2_t<S>* __restrict__ U,
-                                                       const double* __restrict__ Ed_in,
-                                                       const double2* __restrict__ Gp, Geom g,
-                                                       double* __restrict__ Ed_out)
+__global__ __launch_bounds__(kBlock) void k_gauss_fused(const vec2_t<S>* __restrict__ U,
+                                                        const double* __restrict__ Ed_in, Geom g,
+                                                        double* __restrict__ Ed_out,
+                                                        double* __restrict__ partials)
 {
-    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const int i = blockIdx.y;
-    if (s >= g.nsite) return;
-    const Neighbours<D> nb = neighbours<D>(g, s);
-    const mat3<double> G = load_link<double, double>(Gp, g.npad, 0, s);
-    const mat3<double> Gn = load_link<double, double>(Gp, g.npad, 0, s + nb.up[i]);
-    const mat3<double> u = load_link<double, S>(U, g.npad, i, s);
-    mat3<double> a, b, c;
-    mul<false, false>(u, G, a);
-    mul<false, true>(a, u, b);
-    mul<false, false>(b, Gn, c);
-    const double K = 0.12;
+    __shared__ double gl[18][kBlock];
+    double acc[1] = {0.0};
+    const int64_t ntile = (g.nsite + kBlock - 1) / kBlock;
+    for (int64_t t = blockIdx.x; t < ntile; t += gridDim.x) {
+        const int64_t t0 = t * kBlock;
+        const int64_t s = t0 + threadIdx.x;
+        const bool live = s < g.nsite;
+        mat3<double> G;
+        if (live) {
+            gauss_at<D, S>(U, Ed_in, g, s, G);
+            acc[0] += gsum_trace_abs(G);
 #pragma unroll
-    for (int k = 0; k < 9; ++k) c.e[k] = {(c.e[k].re - G.e[k].re) * K, (c.e[k].im - G.e[k].im) * K};
-    double p[8], e[8];
-    re_trace_generators(c, p);
-    load_e<double, double>(Ed_in, g.npad, i, s, e);
+            for (int k = 0; k < 9; ++k) {
+                gl[2 * k][threadIdx.x] = G.e[k].re;
+                gl[2 * k + 1][threadIdx.x] = G.e[k].im;
+            }
+        }
+        __syncthreads();
+        if (live) {
+            const Neighbours<D> nb = neighbours<D>(g, s);
 #pragma unroll
-    for (int a8 = 0; a8 < 8; ++a8) e[a8] = fma(2.0, p[a8], e[a8]);
-    store_e<double, double>(Ed_out, g.npad, i, s, e);
+            for (int i = 0; i < D; ++i) {
+                const int64_t sn = s + nb.up[i];
+                mat3<double> Gn;
+                if (sn >= t0 && sn < t0 + kBlock && sn < g.nsite) {
+                    const int j = (int)(sn - t0);
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) Gn.e[k] = {gl[2 * k][j], gl[2 * k + 1][j]};
+                } else {
+                    gauss_at<D, S>(U, Ed_in, g, sn, Gn);
+                }
+                const mat3<double> u = load_link<double, S>(U, g.npad, i, s);
+                mat3<double> a, b, c;
+                mul<false, false>(u, G, a);
+                mul<false, true>(a, u, b);
+                mul<false, false>(b, Gn, c);
+                const double K = 0.12;
+#pragma unroll
+                for (int k = 0; k < 9; ++k)
+                    c.e[k] = {(c.e[k].re - G.e[k].re) * K, (c.e[k].im - G.e[k].im) * K};
+                double p[8], e[8];
+                re_trace_generators(c, p);
+                load_e<double, double>(Ed_in, g.npad, i, s, e);
+#pragma unroll
+                for (int a8 = 0; a8 < 8; ++a8) e[a8] = fma(2.0, p[a8], e[a8]);
+                store_e<double, double>(Ed_out, g.npad, i, s, e);
+            }
+        }
+        // the next tile overwrites gl
+        __syncthreads();
+    }
+    block_reduce_to<1>(acc, partials + (int64_t)blockIdx.x * kReduceSlots);
 }
 
 // E (storage type) <-> double copy
@@ -172,12 +216,14 @@ hipError_t launch_gauss(const lqcd_lattice* L, const double* Ed, double* d_div)
     return launch_final_reduce1(L, nb, d_div);
 }
 
-hipError_t launch_gauss_step(const lqcd_lattice* L, const double* Ed_in, double* Ed_out)
+hipError_t launch_gauss_fused(const lqcd_lattice* L, const double* Ed_in, double* Ed_out, double* d_div)
 {
-    const dim3 grid(grid_for(L->g.nsite), (unsigned)L->g.dim);
-    LQCD_DISPATCH_DS(L, k_gauss_step<D, S><<<grid, kBlock, 0, L->ctx->stream>>>(
-                            (const vec2_t<S>*)L->U[L->cur], Ed_in, (const double2*)L->gauss_g, L->g, Ed_out));
-    return hipGetLastError();
+    const int nb = gauss_blocks(L->g.nsite);
+    LQCD_DISPATCH_DS(L, k_gauss_fused<D, S><<<nb, kBlock, 0, L->ctx->stream>>>(
+                            (const vec2_t<S>*)L->U[L->cur], Ed_in, L->g, Ed_out, L->partials));
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_final_reduce1(L, nb, d_div);
 }
 
 hipError_t launch_efield_to_double(const lqcd_lattice* L, const void* E, double* Ed)

@@ -126,6 +126,25 @@ __device__ __forceinline__ int64_t class_site(const Geom& g, int64_t k, int pari
     return s0 + ((site_parity<D>(g, s0) + parity) & 1);
 }
 
+// Order of a class launch's workgroups: each covers 2 kBlock consecutive
+// sites (kBlock class links); XCD k gets a contiguous range of them
+// (xcd_remap), walked slab by slab of the slowest direction with that
+// direction fastest (slab_block, when slab_p > 0), so the workgroups whose
+// staples share links run on one XCD close together in time and find them in
+// its L2.  The class index k of a thread does not depend on the order, so the
+// results do not either.
+struct ClassOrder {
+    int64_t slab_p = 0;   // workgroups per slice of the slowest direction (0: XCD remap only)
+    int64_t slab_g = 0;   // slices per XCD slab
+};
+
+__device__ __forceinline__ int64_t class_index(const ClassOrder& co)
+{
+    int64_t logical = xcd_remap(blockIdx.x, gridDim.x);
+    if (co.slab_p > 0) logical = slab_block(logical, co.slab_p, co.slab_g);
+    return logical * kBlock + threadIdx.x;
+}
+
 template <int D, typename S, bool RNG, bool R12 = false>
 __global__ __launch_bounds__(kBlock) void k_metropolis(vec2_t<S>* __restrict__ U, Geom g, int mu,
                                                        int parity, double beta, int n_update,
@@ -133,9 +152,9 @@ __global__ __launch_bounds__(kBlock) void k_metropolis(vec2_t<S>* __restrict__ U
                                                        const uint64_t* __restrict__ uniforms,
                                                        double spread, uint64_t seed, uint64_t counter,
                                                        double* __restrict__ p_out,
-                                                       uint32_t* __restrict__ acc_out)
+                                                       uint32_t* __restrict__ acc_out, ClassOrder co)
 {
-    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t k = class_index(co);
     const int64_t half = g.nsite / 2;
     if (k >= half) return;
     const int64_t s = class_site<D>(g, k, parity);
@@ -327,9 +346,10 @@ __device__ __forceinline__ void sub_block_rc(int grp, int& i, int& j)
 template <int D, typename S>
 __global__ __launch_bounds__(kBlock) void k_heat_bath(vec2_t<S>* __restrict__ U, Geom g, int mu, int parity,
                                                       double beta, uint64_t seed, uint64_t counter,
-                                                      double* __restrict__ p_out, uint32_t* __restrict__ exhausted)
+                                                      double* __restrict__ p_out, uint32_t* __restrict__ exhausted,
+                                                      ClassOrder co)
 {
-    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t k = class_index(co);
     if (k >= g.nsite / 2) return;
     const int64_t s = class_site<D>(g, k, parity);
     mat3<double> a;
@@ -495,9 +515,10 @@ __device__ __forceinline__ void svd3_jacobi(const mat3<double>& a, mat3<double>&
 //   (su3.rs:705-714) negates the off-diagonal entries.  The result does not
 //   depend on the phases or the order of the singular vectors.
 template <int D, typename S, bool REVERSE>
-__global__ __launch_bounds__(kBlock) void k_overrelax(vec2_t<S>* __restrict__ U, Geom g, int mu, int parity)
+__global__ __launch_bounds__(kBlock) void k_overrelax(vec2_t<S>* __restrict__ U, Geom g, int mu, int parity,
+                                                      ClassOrder co)
 {
-    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t k = class_index(co);
     if (k >= g.nsite / 2) return;
     const int64_t s = class_site<D>(g, k, parity);
     mat3<double> a, ad, t, nw;
@@ -528,6 +549,20 @@ __global__ __launch_bounds__(kBlock) void k_overrelax(vec2_t<S>* __restrict__ U,
     store_link<double, S>(U, g.npad, mu, s, nw);
 }
 
+// the slab order of a class launch when the lattice allows it: whole
+// workgroups per slice of the slowest direction, 8 | its extent
+static ClassOrder class_order(const lqcd_lattice* L)
+{
+    ClassOrder co;
+    const int64_t Ls = L->g.extent[L->g.dim - 1];
+    const int64_t slice = L->g.nsite / Ls;
+    if (slice % (2 * kBlock) == 0 && Ls % 8 == 0) {
+        co.slab_p = slice / (2 * kBlock);
+        co.slab_g = Ls / 8;
+    }
+    return co;
+}
+
 hipError_t launch_heat_bath_class(const lqcd_lattice* L, void* U, int mu, int parity, double beta, uint64_t seed,
                                   uint64_t counter)
 {
@@ -535,7 +570,8 @@ hipError_t launch_heat_bath_class(const lqcd_lattice* L, void* U, int mu, int pa
     lqcd_prof_event ev;
     prof_begin(L->ctx, PROF_HEAT_BATH, &ev);
     LQCD_DISPATCH_DS(L, k_heat_bath<D, S><<<grid, kBlock, 0, L->ctx->stream>>>(
-                            (vec2_t<S>*)U, L->g, mu, parity, beta, seed, counter, L->mh_p, L->mh_acc));
+                            (vec2_t<S>*)U, L->g, mu, parity, beta, seed, counter, L->mh_p, L->mh_acc,
+                            class_order(L)));
     prof_end(L->ctx, &ev);
     return hipGetLastError();
 }
@@ -547,10 +583,10 @@ hipError_t launch_overrelax_class(const lqcd_lattice* L, void* U, int mu, int pa
     prof_begin(L->ctx, PROF_OVERRELAX, &ev);
     if (reverse)
         LQCD_DISPATCH_DS(L, k_overrelax<D, S, true><<<grid, kBlock, 0, L->ctx->stream>>>((vec2_t<S>*)U, L->g,
-                                                                                         mu, parity));
+                                                                                         mu, parity, class_order(L)));
     else
         LQCD_DISPATCH_DS(L, k_overrelax<D, S, false><<<grid, kBlock, 0, L->ctx->stream>>>((vec2_t<S>*)U, L->g,
-                                                                                          mu, parity));
+                                                                                          mu, parity, class_order(L)));
     prof_end(L->ctx, &ev);
     return hipGetLastError();
 }
@@ -564,7 +600,7 @@ hipError_t launch_metropolis_class(const lqcd_lattice* L, void* U, int mu, int p
     prof_begin(L->ctx, PROF_METROPOLIS, &ev);
     LQCD_DISPATCH_DS(L, k_metropolis<D, S, false><<<grid, kBlock, 0, L->ctx->stream>>>(
                             (vec2_t<S>*)U, L->g, mu, parity, beta, (int)n_update, d_proposals,
-                            d_uniforms, 0.0, 0, 0, L->mh_p, L->mh_acc));
+                            d_uniforms, 0.0, 0, 0, L->mh_p, L->mh_acc, class_order(L)));
     prof_end(L->ctx, &ev);
     return hipGetLastError();
 }
@@ -586,11 +622,11 @@ hipError_t launch_metropolis_class_rng(const lqcd_lattice* L, void* U, int mu, i
     if (L->md_r12)
         LQCD_DISPATCH_DS(L, k_metropolis<D, S, true, true><<<grid, kBlock, 0, L->ctx->stream>>>(
                                 (vec2_t<S>*)U, L->g, mu, parity, beta, (int)n_update, nullptr, nullptr,
-                                spread, seed, counter, L->mh_p, L->mh_acc));
+                                spread, seed, counter, L->mh_p, L->mh_acc, class_order(L)));
     else
         LQCD_DISPATCH_DS(L, k_metropolis<D, S, true, false><<<grid, kBlock, 0, L->ctx->stream>>>(
                                 (vec2_t<S>*)U, L->g, mu, parity, beta, (int)n_update, nullptr, nullptr,
-                                spread, seed, counter, L->mh_p, L->mh_acc));
+                                spread, seed, counter, L->mh_p, L->mh_acc, class_order(L)));
     prof_end(L->ctx, &ev);
     return hipGetLastError();
 }

@@ -22,6 +22,11 @@
 //                 in the slowest direction run next to each other in time and
 //                 share the XCD's L2 (needs 256 | sites per slice and 8 | L_slow;
 //                 otherwise it falls back to VAR_SITE_XCD's order).
+//   VAR_SITE_TILE per site in VAR_SITE_SLAB's order with an LDS tile of
+//                 two-row f32 links: each workgroup stages its whole x_0 rows
+//                 and the x_1 rows on either side, and reads the links one or
+//                 two steps away in x_0 / x_1 from LDS (20 of the 44 link
+//                 reads of a D = 4 site); the others come from U_in.
 #include "dispatch.hpp"
 #include "md_device.hpp"
 
@@ -85,7 +90,7 @@ __global__ __launch_bounds__(kBlock) void k_md_link(const vec2_t<S>* __restrict_
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int mu = blockIdx.y;
     if (s >= g.nsite) return;
-    md_link<D, S, T, DR, R12>(U_in, U_out, E, g, mu, s, fscale, kick, dfac);
+    md_link<D, S, T, DR, R12, R12 && DR == DRIFT_EXP>(U_in, U_out, E, g, mu, s, fscale, kick, dfac);
 }
 
 // VAR_LINK_XCD: 1-D grid of (site block, mu) pairs, mu fastest, XCD remapped
@@ -99,7 +104,7 @@ __global__ __launch_bounds__(kBlock) void k_md_link_xcd(const vec2_t<S>* __restr
     const int mu = (int)(logical % D);
     const int64_t s = (logical / D) * kBlock + threadIdx.x;
     if (s >= g.nsite) return;
-    md_link<D, S, T, DR, R12>(U_in, U_out, E, g, mu, s, fscale, kick, dfac);
+    md_link<D, S, T, DR, R12, R12 && DR == DRIFT_EXP>(U_in, U_out, E, g, mu, s, fscale, kick, dfac);
 }
 
 // VAR_SITE_XCD / VAR_SITE_SLAB (slab_p > 0): one thread per site, all D links
@@ -117,7 +122,112 @@ __global__ __launch_bounds__(kBlock) void k_md_site_xcd(const vec2_t<S>* __restr
     // workgroups per CU, the occupancy the registers allow anyway)
     constexpr bool OWN = R12 && sizeof(T) == 4;
     __shared__ vec2_t<T> own_lds[OWN ? D * 9 * kBlock : 1];
-    md_site<D, S, T, DR, R12, OWN>(U_in, U_out, E, g, s, fscale, kick, dfac, own_lds);
+    md_site<D, S, T, DR, R12, OWN, R12 && DR == DRIFT_EXP>(U_in, U_out, E, g, s, fscale, kick, dfac, own_lds);
+}
+
+// VAR_SITE_TILE: per site as VAR_SITE_SLAB (same block order), two-row
+// links only: the workgroup first stages rows 0, 1 of the links of its tile
+// and of the two neighbouring x_1 rows in LDS (TileShape), then reads every
+// link reachable by steps in x_0 and x_1 from there and the rest from U_in.
+template <int D, typename S, typename T, int DR>
+__global__ __launch_bounds__(kBlock) void k_md_site_tile(const vec2_t<S>* __restrict__ U_in,
+                                                         vec2_t<S>* __restrict__ U_out,
+                                                         S* __restrict__ E, Geom g, T fscale, T kick,
+                                                         T dfac, int64_t slab_p, int64_t slab_g)
+{
+    extern __shared__ unsigned char tile_raw[];
+    auto* tile = reinterpret_cast<vec2_t<T>*>(tile_raw);
+    int64_t logical = xcd_remap(blockIdx.x, gridDim.x);
+    if (slab_p > 0) logical = slab_block(logical, slab_p, slab_g);
+    TileShape ts;
+    ts.L0 = g.extent[0];
+    ts.rows = kBlock / ts.L0;
+    ts.slots = (ts.rows + 2) * ts.L0;
+    const int64_t t0 = logical * kBlock;
+    const int r = threadIdx.x / ts.L0, c = threadIdx.x % ts.L0;
+    const int64_t s = t0 + threadIdx.x;
+    // the tile's own rows, then the rows x_1 - 1 and x_1 + rows
+#pragma unroll
+    for (int nu = 0; nu < D; ++nu) {
+        const mat3<T> m = load_link_rows01<T, S>(U_in, g.npad, nu, s);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            vec2_t<T> v;
+            v.x = m.e[k].re;
+            v.y = m.e[k].im;
+            tile[(nu * 6 + k) * ts.slots + (r + 1) * ts.L0 + c] = v;
+        }
+    }
+    {
+        const int L1 = g.extent[1];
+        const int64_t x1_0 = (t0 / ts.L0) % L1;
+        const int64_t base = t0 - x1_0 * ts.L0;
+        for (int j = threadIdx.x; j < 2 * ts.L0 * D; j += kBlock) {
+            const int h = j % (2 * ts.L0), nu = j / (2 * ts.L0);
+            const int hr = h / ts.L0, hc = h % ts.L0;
+            const int64_t y1 = hr == 0 ? (x1_0 + L1 - 1) % L1 : (x1_0 + ts.rows) % L1;
+            const mat3<T> m = load_link_rows01<T, S>(U_in, g.npad, nu, base + y1 * ts.L0 + hc);
+            const int slot = (hr == 0 ? 0 : ts.rows + 1) * ts.L0 + hc;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                vec2_t<T> v;
+                v.x = m.e[k].re;
+                v.y = m.e[k].im;
+                tile[(nu * 6 + k) * ts.slots + slot] = v;
+            }
+        }
+    }
+    __syncthreads();
+    const Neighbours<D> nb = neighbours<D>(g, s);
+    const int own = tile_slot(ts, r, c, 0, 0);
+    T f[D][8];
+    {
+        mat3<T> sig[D];
+        staple_sums_site_tile<D, S, T>(U_in, g, s, nb, tile, ts, r, c, sig);
+#pragma unroll
+        for (int mu = 0; mu < D; ++mu) {
+            const mat3<T> u = tile_r12<T>(tile, ts, mu, own);
+            mat3<T> m;
+            mul<false, false>(u, sig[mu], m);
+            im_trace_generators(m, f[mu]);
+        }
+    }
+    T e[D][8];
+#pragma unroll
+    for (int mu = 0; mu < D; ++mu) {
+        load_e<T, S, kCacheStream>(E, g.npad, mu, s, e[mu]);
+#pragma unroll
+        for (int a = 0; a < 8; ++a) e[mu][a] = fma(kick, fscale * f[mu][a], e[mu][a]);
+        store_e<T, S, kCacheStream>(E, g.npad, mu, s, e[mu]);
+    }
+    if constexpr (DR >= 0) {
+#pragma unroll
+        for (int mu = 0; mu < D; ++mu) {
+            const mat3<T> u = tile_r12<T>(tile, ts, mu, own);
+            drift_store<DR, DR == DRIFT_EXP, S, T>(u, e[mu], dfac, U_out, g, mu, s);
+        }
+    }
+}
+
+// whether VAR_SITE_TILE can run on this lattice: two-row f32 links, whole
+// x_0 rows per workgroup that never straddle an x_2 boundary, and an LDS tile
+// of at most 64 KiB
+bool md_tile_fits(const lqcd_lattice* L)
+{
+    const Geom& g = L->g;
+    if (L->precision != 1 || g.dim < 2 || g.nsite % kBlock != 0) return false;
+    const int L0 = g.extent[0];
+    if (L0 < 2 || kBlock % L0 != 0) return false;
+    const int rows = kBlock / L0;
+    if (g.extent[1] % rows != 0 || g.extent[1] < rows) return false;
+    return md_tile_bytes(L) <= 65536;
+}
+
+size_t md_tile_bytes(const lqcd_lattice* L)
+{
+    const int L0 = L->g.extent[0];
+    const size_t slots = (size_t)(kBlock / L0 + 2) * (size_t)L0;
+    return (size_t)L->g.dim * 6 * slots * 2 * sizeof(float);
 }
 
 template <int D, typename S, typename T, int MODE>
@@ -152,12 +262,28 @@ static void launch_md_variant_r(const lqcd_lattice* L, int variant, const void* 
     hipStream_t st = L->ctx->stream;
     const unsigned nsb = grid_for(L->g.nsite);
     switch (variant) {
+    case VAR_SITE_TILE:
+        if constexpr (R12 && sizeof(S) == 4) {
+            if (md_tile_fits(L)) {
+                int64_t slab_p = 0, slab_g = 0;
+                const int64_t Ls = L->g.extent[D - 1];
+                const int64_t slice = L->g.nsite / Ls;
+                if (slice % kBlock == 0 && Ls % 8 == 0) {
+                    slab_p = slice / kBlock;
+                    slab_g = Ls / 8;
+                }
+                k_md_site_tile<D, S, S, DR><<<nsb, kBlock, md_tile_bytes(L), st>>>(ui, uo, e, L->g, fs, k, df,
+                                                                                    slab_p, slab_g);
+                break;
+            }
+        }
+        [[fallthrough]];
     case VAR_SITE_XCD:
     case VAR_SITE_SLAB: {
         int64_t slab_p = 0, slab_g = 0;
         const int64_t Ls = L->g.extent[D - 1];
         const int64_t slice = L->g.nsite / Ls;
-        if (variant == VAR_SITE_SLAB && slice % kBlock == 0 && Ls % 8 == 0) {
+        if (variant != VAR_SITE_XCD && slice % kBlock == 0 && Ls % 8 == 0) {
             slab_p = slice / kBlock;
             slab_g = Ls / 8;
         }

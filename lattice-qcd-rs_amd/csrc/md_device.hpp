@@ -15,24 +15,6 @@
 
 namespace lqcd {
 
-// Workgroups are dispatched round robin over the 8 XCDs (workgroup b runs on
-// XCD b mod 8).  Give XCD k the contiguous logical range
-// [k q + min(k, r), ...) of n = 8 q + r blocks.
-__device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t n)
-{
-    const int64_t k = b & 7, i = b >> 3, q = n >> 3, r = n & 7;
-    return k * q + (k < r ? k : r) + i;
-}
-
-// Site block of logical block j for the slab order: P blocks per slice of the
-// slowest direction, G slices per XCD slab; within a slab the slice index runs
-// fastest.
-__device__ __forceinline__ int64_t slab_block(int64_t j, int64_t P, int64_t G)
-{
-    const int64_t q = j / (G * P), r = j % (G * P);
-    return (q * G + r % G) * P + r / G;
-}
-
 template <int D, typename S, typename T, bool R12 = false>
 __device__ __forceinline__ void staple_sum(const vec2_t<S>* __restrict__ U, const Geom& g, int mu,
                                            int64_t s, const Neighbours<D>& nb, mat3<T>& sigma)
@@ -208,6 +190,109 @@ __device__ __forceinline__ void staple_sums_site(const vec2_t<S>* __restrict__ U
     }
 }
 
+// ---- LDS tile of two-row links (VAR_SITE_TILE) ----
+// A workgroup's kBlock sites are kTileRows = kBlock / L0 whole rows of x_0
+// (L0 = extent[0]) at consecutive x_1 and fixed x_2.., and the tile holds the
+// rows 0, 1 of all D links of those rows and of the rows x_1 - 1 and
+// x_1 + kTileRows (periodic) next to them.  Slot of the site at tile row
+// r in [-1, rows] and x_0 = c: (r + 1) L0 + c; layout [nu][entry 0..5][slot]
+// (consecutive lanes read consecutive words).
+struct TileShape {
+    int L0;      // extent[0], divides kBlock
+    int rows;    // kBlock / L0
+    int slots;   // (rows + 2) * L0
+};
+
+template <typename T>
+__device__ __forceinline__ mat3<T> tile_rows01(const vec2_t<T>* tile, const TileShape& ts, int nu, int slot)
+{
+    mat3<T> m;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const vec2_t<T> v = tile[(nu * 6 + k) * ts.slots + slot];
+        m.e[k] = {v.x, v.y};
+    }
+    return m;
+}
+
+template <typename T>
+__device__ __forceinline__ mat3<T> tile_r12(const vec2_t<T>* tile, const TileShape& ts, int nu, int slot)
+{
+    mat3<T> m = tile_rows01<T>(tile, ts, nu, slot);
+    complete_su3(m);
+    return m;
+}
+
+// slot of the neighbour (d0, d1) of the site at tile row r, x_0 = c
+__device__ __forceinline__ int tile_slot(const TileShape& ts, int r, int c, int d0, int d1)
+{
+    int cc = c + d0;
+    cc = cc < 0 ? cc + ts.L0 : (cc >= ts.L0 ? cc - ts.L0 : cc);
+    return (r + d1 + 1) * ts.L0 + cc;
+}
+
+// staple_sums_site (two-row links, own links from the tile) with every link
+// whose site differs from x in x_0 and x_1 only read from the tile and the
+// others (a step in x_2.. ) from U.
+template <int D, typename S, typename T>
+__device__ __forceinline__ void staple_sums_site_tile(const vec2_t<S>* __restrict__ U, const Geom& g, int64_t s,
+                                                      const Neighbours<D>& nb, const vec2_t<T>* tile,
+                                                      const TileShape& ts, int r, int c, mat3<T> (&sig)[D])
+{
+    // offsets (d0, d1) of a unit step in direction d, and whether it stays in the tile
+    auto in_tile = [](int d) { return d < 2; };
+    auto dx0 = [](int d, int sign) { return d == 0 ? sign : 0; };
+    auto dx1 = [](int d, int sign) { return d == 1 ? sign : 0; };
+    const int own = tile_slot(ts, r, c, 0, 0);
+#pragma unroll
+    for (int mu = 0; mu < D; ++mu) set_zero(sig[mu]);
+#pragma unroll
+    for (int mu = 0; mu < D; ++mu)
+#pragma unroll
+        for (int nu = mu + 1; nu < D; ++nu) {
+            mat3<T> w, st;
+            {
+                const mat3<T> a = in_tile(mu)
+                                      ? tile_rows01<T>(tile, ts, nu, tile_slot(ts, r, c, dx0(mu, 1), dx1(mu, 1)))
+                                      : load_link_rows01<T, S>(U, g.npad, nu, s + nb.up[mu]);
+                const mat3<T> b = in_tile(nu)
+                                      ? tile_r12<T>(tile, ts, mu, tile_slot(ts, r, c, dx0(nu, 1), dx1(nu, 1)))
+                                      : load_link_r12<T, S>(U, g.npad, mu, s + nb.up[nu]);
+                mul_rows01<false, true>(a, b, w);
+                complete_su3(w);
+            }
+            {
+                const mat3<T> cm = tile_r12<T>(tile, ts, nu, own);
+                mul_rows01<false, true>(w, cm, st);
+                add_su3(st, sig[mu]);
+            }
+            {
+                const mat3<T> cm = tile_r12<T>(tile, ts, mu, own);
+                mul_rows01<true, true>(w, cm, st);
+                add_su3(st, sig[nu]);
+            }
+        }
+#pragma unroll
+    for (int nu = 0; nu < D; ++nu) {
+        const int64_t s_mnu = s + nb.dn[nu];
+        const int mslot = tile_slot(ts, r, c, dx0(nu, -1), dx1(nu, -1));
+        const mat3<T> cm = in_tile(nu) ? tile_r12<T>(tile, ts, nu, mslot) : load_link_r12<T, S>(U, g.npad, nu, s_mnu);
+#pragma unroll
+        for (int mu = 0; mu < D; ++mu) {
+            if (mu == nu) continue;
+            const mat3<T> a =
+                (in_tile(nu) && in_tile(mu))
+                    ? tile_r12<T>(tile, ts, nu, tile_slot(ts, r, c, dx0(nu, -1) + dx0(mu, 1), dx1(nu, -1) + dx1(mu, 1)))
+                    : load_link_r12<T, S>(U, g.npad, nu, s_mnu + nb.up[mu]);
+            const mat3<T> b = in_tile(nu) ? tile_r12<T>(tile, ts, mu, mslot) : load_link_r12<T, S>(U, g.npad, mu, s_mnu);
+            mat3<T> ab, st;
+            mul_rows01<true, true>(a, b, ab);
+            mul_rows01<false, false>(ab, cm, st);
+            add_su3(st, sig[mu]);
+        }
+    }
+}
+
 // drift of one link with momentum e:
 //   additive (integrate_link, src/integrator/mod.rs:216-233):
 //       U' = U + dt * (E U) * i sqrt(6) / a
@@ -232,8 +317,38 @@ __device__ __forceinline__ void drift_link(const mat3<T>& u, const T* e, T f, ma
     }
 }
 
+// the exp drift of one link as rows 0 and 1 only (the update stays in SU(3)):
+// exp(iX) U with the product formed for rows 0, 1
+template <typename T>
+__device__ __forceinline__ void drift_link_rows01(const mat3<T>& u, const T* e, T f, mat3<T>& out)
+{
+    T v[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) v[a] = f * e[a];
+    mat3<T> r;
+    exp_i(v, r);
+    mul_rows01<false, false>(r, u, out);
+}
+
+// drift of link (mu, s) into U_out: all nine entries, or with R01 (exp
+// drift of SU(3) links) rows 0, 1 only
+template <int DR, bool R01, typename S, typename T>
+__device__ __forceinline__ void drift_store(const mat3<T>& u, const T* e, T dfac, vec2_t<S>* __restrict__ U_out,
+                                            const Geom& g, int mu, int64_t s)
+{
+    mat3<T> un;
+    if constexpr (R01) {
+        static_assert(DR == DRIFT_EXP, "rows-only stores need the SU(3)-preserving exp drift");
+        drift_link_rows01<T>(u, e, dfac, un);
+        store_link_rows01<T, S, kCacheStream>(U_out, g.npad, mu, s, un);
+    } else {
+        drift_link<T, DR>(u, e, dfac, un);
+        store_link<T, S, kCacheStream>(U_out, g.npad, mu, s, un);
+    }
+}
+
 // what happens with the force of one link (DR: -1 kick only, else drift mode)
-template <int DR, typename S, typename T>
+template <int DR, typename S, typename T, bool R01 = false>
 __device__ __forceinline__ void finish_link(const mat3<T>& u, const mat3<T>& sigma,
                                             vec2_t<S>* __restrict__ U_out, S* __restrict__ E,
                                             const Geom& g, int mu, int64_t s, T fscale, T kick,
@@ -247,16 +362,13 @@ __device__ __forceinline__ void finish_link(const mat3<T>& u, const mat3<T>& sig
 #pragma unroll
     for (int a = 0; a < 8; ++a) e[a] = fma(kick, fscale * f[a], e[a]);
     store_e<T, S, kCacheStream>(E, g.npad, mu, s, e);
-    if (DR >= 0) {
-        mat3<T> un;
-        drift_link<T, DR>(u, e, dfac, un);
-        store_link<T, S, kCacheStream>(U_out, g.npad, mu, s, un);
-    }
+    if constexpr (DR >= 0) drift_store<DR, R01, S, T>(u, e, dfac, U_out, g, mu, s);
 }
 
 // one link (mu, s): e += kick * F(U_in); U_out = drift(U_in, e) unless DR < 0.
-// R12: links read as two rows (load_link_r12), for links in SU(3).
-template <int D, typename S, typename T, int DR, bool R12 = false>
+// R12: links read as two rows (load_link_r12), for links in SU(3); R01: the
+// drifted link is stored as rows 0, 1 only.
+template <int D, typename S, typename T, int DR, bool R12 = false, bool R01 = false>
 __device__ __forceinline__ void md_link(const vec2_t<S>* __restrict__ U_in, vec2_t<S>* __restrict__ U_out,
                                         S* __restrict__ E, const Geom& g, int mu, int64_t s, T fscale,
                                         T kick, T dfac)
@@ -265,11 +377,11 @@ __device__ __forceinline__ void md_link(const vec2_t<S>* __restrict__ U_in, vec2
     mat3<T> sigma;
     staple_sum<D, S, T, R12>(U_in, g, mu, s, nb, sigma);
     const mat3<T> u = load_link_sel<R12, T, S>(U_in, g.npad, mu, s);
-    finish_link<DR, S, T>(u, sigma, U_out, E, g, mu, s, fscale, kick, dfac);
+    finish_link<DR, S, T, R01>(u, sigma, U_out, E, g, mu, s, fscale, kick, dfac);
 }
 
 // all D links of site s, sharing the staple loads
-template <int D, typename S, typename T, int DR, bool R12 = false, bool OWN = false>
+template <int D, typename S, typename T, int DR, bool R12 = false, bool OWN = false, bool R01 = false>
 __device__ __forceinline__ void md_site(const vec2_t<S>* __restrict__ U_in, vec2_t<S>* __restrict__ U_out,
                                         S* __restrict__ E, const Geom& g, int64_t s, T fscale, T kick,
                                         T dfac, vec2_t<T>* own = nullptr)
@@ -303,13 +415,11 @@ __device__ __forceinline__ void md_site(const vec2_t<S>* __restrict__ U_in, vec2
         store_e<T, S, kCacheStream>(E, g.npad, mu, s, e[mu]);
     }
     // phase 3: drifts (U_mu(x) is read again, from cache)
-    if (DR >= 0) {
+    if constexpr (DR >= 0) {
 #pragma unroll
         for (int mu = 0; mu < D; ++mu) {
             const mat3<T> u = OWN ? own_get<T>(own, mu) : load_link_sel<R12, T, S>(U_in, g.npad, mu, s);
-            mat3<T> un;
-            drift_link<T, DR>(u, e[mu], dfac, un);
-            store_link<T, S, kCacheStream>(U_out, g.npad, mu, s, un);
+            drift_store<DR, R01, S, T>(u, e[mu], dfac, U_out, g, mu, s);
         }
     }
 }

@@ -13,6 +13,7 @@ Arrays use the reference's in-memory layouts:
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -517,6 +518,17 @@ def bernoulli_accept(p: float, u: int) -> bool:
 
 def clamp_probability(x: float) -> float:
     return float(lib().or_clamp_probability(x))
+
+
+def replacement_probability(h_old: float, h_new: float) -> float:
+    """(h_old - h_new).exp().min(1).max(0) of hybrid_monte_carlo.rs:584-589
+    with f64 exp semantics: an overflow is +inf (so p = 1), NaN stays NaN
+    (so p = 1 through Rust's min)."""
+    try:
+        e = math.exp(h_old - h_new)
+    except OverflowError:
+        e = math.inf
+    return clamp_probability(e)
 
 
 def factorial_size_for_exp() -> int:

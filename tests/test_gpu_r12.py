@@ -144,3 +144,30 @@ def test_metropolis_two_row_staples():
         out[mode] = (st.md_reconstruct_last(), st.link_matrix())
     assert not out[1][0]
     assert np.max(np.abs(out[1][1] - out[0][1])) <= 1e-12
+
+
+@pytest.mark.parametrize("dim,ext", [(4, (32, 8, 4, 8)), (4, (16, 16, 4, 4)), (4, (8, 32, 4, 4)), (3, (16, 16, 4)),
+                                     (2, (16, 16)), (4, (4, 4, 4, 6))])
+def test_tile_variant_matches_site_variant(dim, ext):
+    # MD variant 4 (LDS tile of two-row f32 links over whole x_0 rows and the
+    # x_1 rows either side) against variant 3 (the same per-site arithmetic
+    # with every link read from HBM) and the f64 oracle on f32-rounded
+    # inputs; (4, 4, 4, 6) has no whole-row tile (16 sites per x_1 column):
+    # variant 4 falls back to variant 3 there
+    announce(f"MD tile variant ext={ext}")
+    ol = O.OracleLattice(dim, list(ext))
+    u = ol.hot_start(21).astype(np.float32).astype(np.float64)
+    e = ol.random_efield(22, 0.5 / 6.0).astype(np.float32).astype(np.float64)
+    got = {}
+    for variant in (3, 4):
+        st = lqcd.LatticeStateDefault(lqcd.LatticeCyclic(1.0, list(ext), dim), 6.0, u, lqcd.F32)
+        st.set_md_variant(variant)
+        ef = lqcd.LatticeStateEFSyncDefault(st, e)
+        ef.simulate_symplectic_n(lqcd.SymplecticEulerGpu(lqcd.DRIFT_EXP, lqcd.SCHEME_LEAPFROG), 0.01, 6)
+        assert st.md_reconstruct_last() and st.md_variant() == variant
+        got[variant] = (ef.link_matrix(), ef.e_field())
+    assert np.max(np.abs(got[4][0] - got[3][0])) <= 1e-6
+    assert np.max(np.abs(got[4][1] - got[3][1])) <= 1e-6
+    u_ref, e_ref = ol.simulate_symplectic_n(u, e, 0.01, 6, O.DRIFT_EXP, O.SCHEME_LEAPFROG)
+    assert np.max(np.abs(got[4][0] - u_ref)) <= 1e-5
+    assert np.max(np.abs(got[4][1] - e_ref)) <= 1e-5
