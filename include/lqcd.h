@@ -488,6 +488,19 @@ int lqcd_stats_mean(lqcd_ctx* ctx, const double* d_values, size_t n, int complex
  * underflows); n = 1 gives a NaN variance, as in the reference. */
 int lqcd_stats_mean_and_variance(lqcd_ctx* ctx, const double* d_values, size_t n, int complex_values, int form,
                                  double* out);
+/* lqcd_stats_mean_and_variance with no device allocation: no counterpart in
+ * the reference.  For callers that cannot allocate device memory (code under
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc); the
+ * plain entry point allocates its scratch on a context's first statistics
+ * call.  Same arguments, results and errors.  The block partials and sums
+ * live in one fixed device array per GPU (192 KiB + 128 B, held until the
+ * process ends), whose address is resolved when a context is created on that
+ * GPU.  Every value the kernels read from it is written earlier in the same
+ * call, so nothing needs clearing; calls that overlap on one GPU (two host
+ * threads, or two contexts' streams) race on it.  n above 2^62 is
+ * LQCD_ERR_INVALID_ARGUMENT. */
+int lqcd_stats_mean_and_variance_noalloc(lqcd_ctx* ctx, const double* d_values, size_t n, int complex_values,
+                                         int form, double* out);
 /* mean_with_error / mean_with_error_par_iter_val (mod.rs:401-405, 201-207):
  * out = mean, sqrt(variance / len), real values only. */
 int lqcd_stats_mean_with_error(lqcd_ctx* ctx, const double* d_values, size_t n, int form, double out[2]);

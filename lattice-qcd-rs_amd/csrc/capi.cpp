@@ -353,6 +353,8 @@ int lqcd_ctx_create(int device, lqcd_ctx** out)
         delete ctx;
         return fail(LQCD_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
+    // the allocation-free statistics entry point retries a failure here
+    (void)stats_fixed_scratch_init(device);
     *out = ctx;
     return LQCD_OK;
 }

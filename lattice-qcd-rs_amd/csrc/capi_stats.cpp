@@ -9,6 +9,8 @@
 // "variance" is the mean of (x - m)(x - m).
 #include <cmath>
 #include <complex>
+#include <cstdint>
+#include <mutex>
 
 #include "capi_util.hpp"
 #include "internal.hpp"
@@ -18,11 +20,25 @@ using namespace lqcd;
 // products and quotients rounded one operation at a time, as in the reference
 #pragma clang fp contract(off)
 
+constexpr int kStatsSlots = 6;   // doubles per block partial (complex covariance)
+// doubles of one call's device scratch: the block partials (at most
+// kMaxReduceBlocks blocks, stats_blocks caps the grid there for any n), then
+// two vectors of 8 sums
+constexpr size_t kScratchDoubles = (size_t)kMaxReduceBlocks * kStatsSlots + 16;
+// largest n of the allocation-free entry point: element indices (2 i for
+// complex values) stay within int64
+constexpr size_t kNoAllocMaxValues = (size_t)1 << 62;
+
+// The fixed scratch of lqcd_stats_mean_and_variance_noalloc: one copy per
+// GPU, held until the process ends (192 KiB + 128 B).  In host code this name
+// is a host-side stand-in, not a device address: each device's address comes
+// from hipGetSymbolAddress (resolve_fixed_scratch, once per device with that
+// device current), and only that address reaches kernels and copies.
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
 namespace {
 
 using cd = std::complex<double>;
-
-constexpr int kStatsSlots = 6;   // doubles per block partial (complex covariance)
 
 // num-complex product, component by component as the reference computes it
 cd cmul(cd a, cd b)
@@ -39,13 +55,39 @@ int check_form(int form)
 }
 
 // the partials and the two sum vectors of one call, kept on the context
-int scratch(lqcd_ctx* ctx, double** partials, double** sums)
+// (allocated on the context's first statistics call)
+int ctx_scratch(lqcd_ctx* ctx, double** partials, double** sums)
 {
-    if (!ctx->stats_scratch)
-        HIP_TRY(hipMalloc((void**)&ctx->stats_scratch, sizeof(double) * (kMaxReduceBlocks * kStatsSlots + 16)));
+    if (!ctx->stats_scratch) HIP_TRY(hipMalloc((void**)&ctx->stats_scratch, sizeof(double) * kScratchDoubles));
     *partials = ctx->stats_scratch;
     *sums = ctx->stats_scratch + kMaxReduceBlocks * kStatsSlots;
     return LQCD_OK;
+}
+
+// per-device addresses of the fixed scratch
+constexpr int kMaxDevices = 64;
+std::mutex fixed_mu;
+double* fixed_ptr[kMaxDevices] = {};
+
+// resolves the address of `scratch` on `device`, which must be current
+hipError_t resolve_fixed_scratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(fixed_mu);
+    if (fixed_ptr[device]) return hipSuccess;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    fixed_ptr[device] = (double*)p;
+    return hipSuccess;
+}
+
+double* fixed_scratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(fixed_mu);
+    return fixed_ptr[device];
 }
 
 // element k of a sum vector read back from the device
@@ -59,11 +101,9 @@ void put(cd v, bool cplx, double* out)
 
 // runs op over n values and copies its sums (and, for the two-pass variance,
 // the centred sum after them) to h
-int device_sums(lqcd_ctx* ctx, int op, bool cplx, const double* x, const double* y, size_t n, bool centred,
-                double* h)
+int device_sums_in(lqcd_ctx* ctx, double* partials, double* sums, int op, bool cplx, const double* x,
+                   const double* y, size_t n, bool centred, double* h)
 {
-    double *partials, *sums;
-    if (int rc = scratch(ctx, &partials, &sums)) return rc;
     const int nb = stats_blocks((int64_t)n);
     HIP_TRY(launch_stats(ctx, op, cplx, x, y, (int64_t)n, nullptr, partials, nb, sums));
     if (centred)
@@ -73,7 +113,18 @@ int device_sums(lqcd_ctx* ctx, int op, bool cplx, const double* x, const double*
     return LQCD_OK;
 }
 
-int mean_variance(lqcd_ctx* ctx, const double* x, size_t n, bool cplx, int form, cd* mean, cd* var)
+int device_sums(lqcd_ctx* ctx, int op, bool cplx, const double* x, const double* y, size_t n, bool centred,
+                double* h)
+{
+    double *partials, *sums;
+    if (int rc = ctx_scratch(ctx, &partials, &sums)) return rc;
+    return device_sums_in(ctx, partials, sums, op, cplx, x, y, n, centred, h);
+}
+
+// mean and variance of n values; fixed = the partials and sums live in this
+// device's copy of `scratch` instead of the context's scratch
+int mean_variance(lqcd_ctx* ctx, const double* x, size_t n, bool cplx, int form, cd* mean, cd* var,
+                  bool fixed = false)
 {
     CHECK_ARG(ctx, "ctx is null");
     if (int rc = check_form(form)) return rc;
@@ -83,7 +134,24 @@ int mean_variance(lqcd_ctx* ctx, const double* x, size_t n, bool cplx, int form,
     DeviceGuard dg(ctx->device);
     double h[16] = {};
     const bool two_pass = form == LQCD_STATS_SEQUENTIAL;
-    if (int rc = device_sums(ctx, STATS_MOMENTS, cplx, x, nullptr, n, two_pass, h)) return rc;
+    if (fixed) {
+        CHECK_ARG(n <= kNoAllocMaxValues, "more values than lqcd_stats_mean_and_variance_noalloc accepts (2^62)");
+        double* base = fixed_scratch(ctx->device);
+        if (!base) {
+            const hipError_t e = resolve_fixed_scratch(ctx->device);
+            if (e != hipSuccess)
+                return fail(LQCD_ERR_DEVICE, std::string("fixed statistics scratch: ") + hipGetErrorString(e));
+            base = fixed_scratch(ctx->device);
+        }
+        // the grid never has more blocks than the scratch holds partials for
+        CHECK_ARG((size_t)stats_blocks((int64_t)n) * kStatsSlots + 16 <= kScratchDoubles,
+                  "statistics grid larger than the fixed scratch");
+        if (int rc = device_sums_in(ctx, base, base + (size_t)kMaxReduceBlocks * kStatsSlots, STATS_MOMENTS, cplx,
+                                    x, nullptr, n, two_pass, h))
+            return rc;
+    } else if (int rc = device_sums(ctx, STATS_MOMENTS, cplx, x, nullptr, n, two_pass, h)) {
+        return rc;
+    }
     const double len = (double)n;
     const cd s = sum_at(h, 0, cplx);
     *mean = cdiv(s, len);
@@ -97,6 +165,11 @@ int mean_variance(lqcd_ctx* ctx, const double* x, size_t n, bool cplx, int form,
 }
 
 } // namespace
+
+// resolves this device's copy of the fixed statistics scratch at context
+// creation (the device is current); a failure here is reported again by the
+// first allocation-free call, which retries
+hipError_t lqcd::stats_fixed_scratch_init(int device) { return resolve_fixed_scratch(device); }
 
 extern "C" {
 
@@ -124,6 +197,18 @@ int lqcd_stats_mean_and_variance(lqcd_ctx* ctx, const double* d_values, size_t n
     const bool cplx = complex_values != 0;
     cd m, v;
     if (int rc = mean_variance(ctx, d_values, n, cplx, form, &m, &v)) return rc;
+    put(m, cplx, out);
+    put(v, cplx, out + (cplx ? 2 : 1));
+    return LQCD_OK;
+}
+
+int lqcd_stats_mean_and_variance_noalloc(lqcd_ctx* ctx, const double* d_values, size_t n, int complex_values,
+                                         int form, double* out)
+{
+    CHECK_ARG(out, "null argument");
+    const bool cplx = complex_values != 0;
+    cd m, v;
+    if (int rc = mean_variance(ctx, d_values, n, cplx, form, &m, &v, true)) return rc;
     put(m, cplx, out);
     put(v, cplx, out + (cplx ? 2 : 1));
     return LQCD_OK;

@@ -205,6 +205,10 @@ int stats_blocks(int64_t n);
 hipError_t launch_stats(lqcd_ctx* ctx, int op, bool cplx, const double* x, const double* y, int64_t n,
                         const double* center, double* partials, int nblocks, double* d_out);
 
+// resolves `device`'s address of the fixed statistics scratch (capi_stats.cpp);
+// the device must be current
+hipError_t stats_fixed_scratch_init(int device);
+
 // profiling hooks (capi.cpp)
 void prof_begin(lqcd_ctx* ctx, int kind, lqcd_prof_event* ev);
 void prof_end(lqcd_ctx* ctx, lqcd_prof_event* ev);

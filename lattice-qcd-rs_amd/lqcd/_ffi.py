@@ -150,6 +150,7 @@ SIGNATURES = {
     "lqcd_field_strength_device": (_I, [_P, _I, _P, _S]),
     "lqcd_stats_mean": (_I, [_P, _P, _S, _I, _P]),
     "lqcd_stats_mean_and_variance": (_I, [_P, _P, _S, _I, _I, _P]),
+    "lqcd_stats_mean_and_variance_noalloc": (_I, [_P, _P, _S, _I, _I, _P]),
     "lqcd_stats_mean_with_error": (_I, [_P, _P, _S, _I, _P]),
     "lqcd_stats_covariance": (_I, [_P, _P, _S, _P, _S, _I, _I, _P, C.POINTER(C.c_int)]),
     "lqcd_md_reconstruct_set": (_I, [_P, _I]),
