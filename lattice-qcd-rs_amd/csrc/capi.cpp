@@ -14,6 +14,7 @@
 
 #include "../../include/lqcd.h"
 #include "capi_util.hpp"
+#include "dispatch.hpp"
 #include "internal.hpp"
 
 using namespace lqcd;
@@ -156,12 +157,15 @@ int links_su3(lqcd_lattice* L, int64_t updates, bool* ok)
 //             event pair spans the replay and counts n fused launches.
 //   REFERENCE: per step a fused kick(dt/2)+drift(dt) and a kick(dt/2); used
 //             only with profiling off (the replay mixes two kernel kinds).
-// One graph per starting buffer; all are captured again when a parameter
-// they bake in changes.
-static int md_graph_replay(lqcd_lattice* L, double dt, size_t n, int drift, int scheme)
+// `first` goes with the first fused launch and, for REFERENCE, `last` with
+// the final kick (MdExtra; null: none).  One graph per starting buffer; all
+// are captured again when a parameter they bake in changes.
+static int md_graph_replay(lqcd_lattice* L, double dt, size_t n, int drift, int scheme, const MdExtra* first,
+                           const MdExtra* last)
 {
+    const int extras = (first ? 1 : 0) | (last && last->row2 ? 2 : 0) | (last && last->sums ? 4 : 0);
     if (L->mg_steps != n || L->mg_dt != dt || L->mg_drift != drift || L->mg_variant != L->md_variant ||
-        L->mg_r12 != L->md_r12 || L->mg_scheme != scheme) {
+        L->mg_r12 != L->md_r12 || L->mg_scheme != scheme || L->mg_extras != extras) {
         for (auto& ge : L->md_graph)
             if (ge) {
                 (void)hipGraphExecDestroy(ge);
@@ -173,6 +177,7 @@ static int md_graph_replay(lqcd_lattice* L, double dt, size_t n, int drift, int 
         L->mg_variant = L->md_variant;
         L->mg_r12 = L->md_r12;
         L->mg_scheme = scheme;
+        L->mg_extras = extras;
     }
     hipGraphExec_t& exec = L->md_graph[L->cur];
     hipStream_t st = L->ctx->stream;
@@ -186,11 +191,12 @@ static int md_graph_replay(lqcd_lattice* L, double dt, size_t n, int drift, int 
         hipError_t le = hipSuccess;
         int src = L->cur, dst = w1;
         for (size_t k = 0; k < n && le == hipSuccess; ++k) {
+            const MdExtra* x = k == 0 ? first : nullptr;
             if (scheme == LQCD_SCHEME_REFERENCE) {
-                le = launch_kick_drift(L, L->U[src], L->U[dst], L->E, dt / 2.0, dt, drift);
-                if (le == hipSuccess) le = launch_kick(L, L->U[dst], L->E, dt / 2.0);
+                le = launch_kick_drift(L, L->U[src], L->U[dst], L->E, dt / 2.0, dt, drift, x);
+                if (le == hipSuccess) le = launch_kick(L, L->U[dst], L->E, dt / 2.0, k + 1 == n ? last : nullptr);
             } else {
-                le = launch_kick_drift(L, L->U[src], L->U[dst], L->E, k == 0 ? dt / 2.0 : dt, dt, drift);
+                le = launch_kick_drift(L, L->U[src], L->U[dst], L->E, k == 0 ? dt / 2.0 : dt, dt, drift, x);
             }
             src = dst;
             dst = (dst == w1) ? w2 : w1;
@@ -231,8 +237,15 @@ static int buffer_after(const lqcd_lattice* L, size_t n)
 // holding the final links.  U[cur] is left untouched.  An integration with
 // the exp drift keeps the links in SU(3) (up to rounding), so when the
 // starting links are in SU(3) its kernels read only their first two rows and
-// store only the first two rows of the links they drift.
-int integrate_steps(lqcd_lattice* L, double dt, size_t n, int drift, int scheme, int* last)
+// store only the first two rows of the links they drift; the final kick of
+// the per-site kernels writes row 2 of the final links back (otherwise
+// k_complete_row2 does).  sums (non-null): the terms of H before and after
+// the integration go to result.  With a per-site work decomposition the
+// first launch and the final kick sum them on the way (MdExtra): result[0..3]
+// = sum Re Tr(U Sigma) and sum e^2 over all links, before, then after, and
+// *sums = true.  Otherwise the plaquette kernel runs before and after the MD
+// (launch_hamiltonian_sum into result[0..3] and result[4..7]), *sums = false.
+int integrate_steps(lqcd_lattice* L, double dt, size_t n, int drift, int scheme, int* last, bool* sums = nullptr)
 {
     R12Scope scope{L};
     L->md_r12 = false;
@@ -245,37 +258,67 @@ int integrate_steps(lqcd_lattice* L, double dt, size_t n, int drift, int scheme,
     if (drift == LQCD_DRIFT_ADDITIVE) L->su3_known = -2;
     L->md_r12_last = L->md_r12;
     if (int rc = ensure_md_variant(L, drift)) return rc;
+    const bool site = md_variant_has_sums(L->md_variant);
+    const bool with_sums = sums && site && n > 0;
+    const int64_t nb = (int64_t)grid_for(L->g.nsite);
+    if (with_sums) {
+        if (!L->md_partials)
+            HIP_TRY(hipMalloc((void**)&L->md_partials, sizeof(double) * 2 * (size_t)nb * kReduceSlots));
+    } else if (sums) {
+        HIP_TRY(launch_hamiltonian_sum(L, L->U[L->cur], L->E, L->result));
+    }
+    const int final_buf = buffer_after(L, n);
+    MdExtra first, fin;
+    if (with_sums) {
+        first.sums = 1;
+        first.partials = L->md_partials;
+        fin.sums = 2;
+        fin.partials = L->md_partials + nb * kReduceSlots;
+    }
+    // the final kick reads the final links as two rows and completes them
+    // anyway: it writes their row 2
+    if (site && L->md_r12 && n > 0) fin.row2 = L->U[final_buf];
+    const MdExtra* px = with_sums ? &first : nullptr;
+    const MdExtra* pf = (with_sums || fin.row2) ? &fin : nullptr;
     int w1, w2;
     other_buffers(L->cur, &w1, &w2);
     int src = L->cur, dst = w1;
     if (scheme == LQCD_SCHEME_REFERENCE) {
         if (L->md_graphs && !L->ctx->profiling) {
-            if (int rc = md_graph_replay(L, dt, n, drift, scheme)) return rc;
-            src = buffer_after(L, n);
+            if (int rc = md_graph_replay(L, dt, n, drift, scheme, px, pf)) return rc;
+            src = final_buf;
         } else {
             for (size_t k = 0; k < n; ++k) {
-                HIP_TRY(launch_kick_drift(L, L->U[src], L->U[dst], L->E, dt / 2.0, dt, drift));
-                HIP_TRY(launch_kick(L, L->U[dst], L->E, dt / 2.0));
+                HIP_TRY(launch_kick_drift(L, L->U[src], L->U[dst], L->E, dt / 2.0, dt, drift, k == 0 ? px : nullptr));
+                HIP_TRY(launch_kick(L, L->U[dst], L->E, dt / 2.0, k + 1 == n ? pf : nullptr));
                 src = dst;
                 dst = (dst == w1) ? w2 : w1;
             }
         }
     } else {
         if (L->md_graphs) {
-            if (int rc = md_graph_replay(L, dt, n, drift, scheme)) return rc;
-            src = buffer_after(L, n);
+            if (int rc = md_graph_replay(L, dt, n, drift, scheme, px, nullptr)) return rc;
+            src = final_buf;
         } else {
             for (size_t k = 0; k < n; ++k) {
-                HIP_TRY(launch_kick_drift(L, L->U[src], L->U[dst], L->E, k == 0 ? dt / 2.0 : dt, dt, drift));
+                HIP_TRY(launch_kick_drift(L, L->U[src], L->U[dst], L->E, k == 0 ? dt / 2.0 : dt, dt, drift,
+                                          k == 0 ? px : nullptr));
                 src = dst;
                 dst = (dst == w1) ? w2 : w1;
             }
         }
-        HIP_TRY(launch_kick(L, L->U[src], L->E, dt / 2.0));
+        HIP_TRY(launch_kick(L, L->U[src], L->E, dt / 2.0, pf));
     }
     // two-row kernels store the drifted links as rows 0, 1: the final links
     // get their row 2 back before anything else reads them
-    if (L->md_r12 && n > 0) HIP_TRY(launch_complete_row2(L, L->U[src]));
+    if (L->md_r12 && n > 0 && !fin.row2) HIP_TRY(launch_complete_row2(L, L->U[src]));
+    if (with_sums) {
+        HIP_TRY(launch_final_reduce_at(L, first.partials, (int)nb, L->result));
+        HIP_TRY(launch_final_reduce_at(L, fin.partials, (int)nb, L->result + 2));
+    } else if (sums) {
+        HIP_TRY(launch_hamiltonian_sum(L, L->U[src], L->E, L->result + 4));
+    }
+    if (sums) *sums = with_sums;
     *last = src;
     return LQCD_OK;
 }
@@ -470,6 +513,7 @@ int lqcd_lattice_destroy(lqcd_lattice* L)
     if (L->mh_p) (void)hipFree(L->mh_p);
     if (L->mh_acc) (void)hipFree(L->mh_acc);
     if (L->gauss_g) (void)hipFree(L->gauss_g);
+    if (L->md_partials) (void)hipFree(L->md_partials);
     for (int k = 0; k < 3; ++k)
         if (L->gauss_e[k]) (void)hipFree(L->gauss_e[k]);
     for (auto& ge : L->md_graph)
@@ -970,15 +1014,28 @@ int lqcd_hmc_trajectory(lqcd_lattice* L, double beta, double dt, size_t n, int d
     CHECK_ARG(drift == LQCD_DRIFT_ADDITIVE || drift == LQCD_DRIFT_EXP, "unknown drift mode");
     CHECK_ARG(scheme == LQCD_SCHEME_REFERENCE || scheme == LQCD_SCHEME_LEAPFROG, "unknown scheme");
     DeviceGuard dg(L->ctx->device);
-    // H_old's sums stay on the device (result[0..3]) until H_new's
-    // (result[4..7]) are done: one host round trip per trajectory
-    HIP_TRY(launch_hamiltonian_sum(L, L->U[L->cur], L->E, L->result));
+    // The per-site MD kernels sum H_old's and H_new's terms on the way (the
+    // first launch over the starting links and momenta, the final kick over
+    // the final ones; integrate_steps): every plaquette enters
+    // sum Re Tr(U Sigma) through its 4 links, so
+    // H_U = beta (N_plaquettes - sum Re Tr(U Sigma) / 12), H_E = beta sum e^2 / 2.
+    // Other work decompositions use the plaquette kernel before and after.
+    // Either way the sums stay on the device until the MD is done: one host
+    // round trip per trajectory.
     int last = L->cur;
-    if (int rc = integrate_steps(L, dt, n, drift, scheme, &last)) return rc;
-    HIP_TRY(launch_hamiltonian_sum(L, L->U[last], L->E, L->result + 4));
-    if (int rc = read_results(L, 8)) return rc;
-    const double h_old = beta * L->host_result[2] + beta * L->host_result[3];
-    const double h_new = beta * L->host_result[6] + beta * L->host_result[7];
+    bool fused = false;
+    if (int rc = integrate_steps(L, dt, n, drift, scheme, &last, &fused)) return rc;
+    double h_old, h_new;
+    if (fused) {
+        if (int rc = read_results(L, 4)) return rc;
+        const double nplaq = (double)L->g.nsite * (double)(L->g.dim * (L->g.dim - 1) / 2);
+        h_old = beta * (nplaq - L->host_result[0] / 12.0) + beta * (0.5 * L->host_result[1]);
+        h_new = beta * (nplaq - L->host_result[2] / 12.0) + beta * (0.5 * L->host_result[3]);
+    } else {
+        if (int rc = read_results(L, 8)) return rc;
+        h_old = beta * L->host_result[2] + beta * L->host_result[3];
+        h_new = beta * L->host_result[6] + beta * L->host_result[7];
+    }
     // probability_of_replacement, hybrid_monte_carlo.rs:584-589 (clamped twice)
     double p = std::fmax(std::fmin(std::exp(h_old - h_new), 1.0), 0.0);
     p = std::fmax(std::fmin(p, 1.0), 0.0);

@@ -53,7 +53,7 @@ struct lqcd_lattice {
     hipGraphExec_t md_graph[3] = {nullptr, nullptr, nullptr};
     size_t mg_steps = 0;
     double mg_dt = 0.0;
-    int mg_drift = -1, mg_variant = -1, mg_scheme = -1;
+    int mg_drift = -1, mg_variant = -1, mg_scheme = -1, mg_extras = -1;
     bool mg_r12 = false;
     // what is known of the distance of U[cur] from SU(3): >= 0 within
     // r12_distance, counting the SU(3)-preserving updates since that was
@@ -61,6 +61,7 @@ struct lqcd_lattice {
     int64_t su3_known = -1;
     void* gauss_g = nullptr;       // Gauss law G(x): 9 planes of double2
     double* gauss_e[3] = {nullptr, nullptr, nullptr};  // double copies of E for the projection
+    double* md_partials = nullptr;  // the HMC's fused H sums: two sets of per-workgroup partials
 };
 
 namespace lqcd {
@@ -138,18 +139,31 @@ enum MdVariant {
 // VAR_SITE_TILE's LDS tile: whether it applies to the lattice, and its bytes
 bool md_tile_fits(const lqcd_lattice* L);
 size_t md_tile_bytes(const lqcd_lattice* L);
-hipError_t launch_kick_variant(const lqcd_lattice* L, int variant, const void* U, void* E, double kick);
+// What an MD launch of the per-site kernels (variants 2-4) does besides its
+// kick (+ drift): sums (MdSums in md_device.hpp: 0 none, 1 before the kick,
+// 2 after it) of Re Tr(U Sigma) and e^2 into partials[block][0..1] (a
+// workgroup per 256 sites), and with two-row reads row 2 of the links read
+// written to row2 (null: not).
+struct MdExtra {
+    int sums = 0;
+    double* partials = nullptr;
+    void* row2 = nullptr;
+};
+// whether an MD work decomposition takes MdExtra (the per-site kernels)
+bool md_variant_has_sums(int variant);
+hipError_t launch_kick_variant(const lqcd_lattice* L, int variant, const void* U, void* E, double kick,
+                               const MdExtra* x = nullptr);
 hipError_t launch_kick_drift_variant(const lqcd_lattice* L, int variant, const void* U_in, void* U_out,
-                                     void* E, double kick, double dt, int drift_mode);
+                                     void* E, double kick, double dt, int drift_mode, const MdExtra* x = nullptr);
 // out[(s*D+mu)*8+a] = scale * derivative_e^a
 hipError_t launch_force_store(const lqcd_lattice* L, const void* U, double* d_out, double scale);
 // derivative_u of every link of U[cur] with E, host layout (18 doubles per link)
 hipError_t launch_derivative_u(const lqcd_lattice* L, double* d_out);
 // E += kick * derivative_e(U)
-hipError_t launch_kick(const lqcd_lattice* L, const void* U, void* E, double kick);
+hipError_t launch_kick(const lqcd_lattice* L, const void* U, void* E, double kick, const MdExtra* x = nullptr);
 // E += kick * derivative_e(U_in); U_out = drift(U_in, E, dt)
 hipError_t launch_kick_drift(const lqcd_lattice* L, const void* U_in, void* U_out, void* E, double kick,
-                             double dt, int drift_mode);
+                             double dt, int drift_mode, const MdExtra* x = nullptr);
 // U_out = drift(U_in, E, dt)
 hipError_t launch_drift(const lqcd_lattice* L, const void* U_in, void* U_out, const void* E, double dt,
                         int drift_mode);
@@ -191,6 +205,8 @@ hipError_t launch_efield_to_double(const lqcd_lattice* L, const void* E, double*
 hipError_t launch_efield_from_double(const lqcd_lattice* L, const double* Ed, void* E);
 // sum of nblocks partials (slot 0) into d_out1
 hipError_t launch_final_reduce1(const lqcd_lattice* L, int nblocks, double* d_out1);
+// slots 0, 1 of nblocks partials at `partials`, each summed in block order, into d_out2[0..1]
+hipError_t launch_final_reduce_at(const lqcd_lattice* L, const double* partials, int nblocks, double* d_out2);
 // max over links of max(|U U^+ - 1|_ij, |det U - 1|) of buffer U into d_out1
 hipError_t launch_su3_distance(const lqcd_lattice* L, const void* U, double* d_out1);
 

@@ -29,6 +29,7 @@
 //                 reads of a D = 4 site); the others come from U_in.
 #include "dispatch.hpp"
 #include "md_device.hpp"
+#include "observables_device.hpp"
 
 namespace lqcd {
 
@@ -107,33 +108,49 @@ __global__ __launch_bounds__(kBlock) void k_md_link_xcd(const vec2_t<S>* __restr
     md_link<D, S, T, DR, R12, R12 && DR == DRIFT_EXP>(U_in, U_out, E, g, mu, s, fscale, kick, dfac);
 }
 
-// VAR_SITE_XCD / VAR_SITE_SLAB (slab_p > 0): one thread per site, all D links
-template <int D, typename S, typename T, int DR, bool R12>
-__global__ __launch_bounds__(kBlock) void k_md_site_xcd(const vec2_t<S>* __restrict__ U_in,
+// VAR_SITE_XCD / VAR_SITE_SLAB (slab_p > 0): one thread per site, all D links.
+// SUMS: the workgroup's Hamiltonian sums (MdSums) go to partials[blockIdx.x];
+// row2: see md_site.
+template <int D, typename S, typename T, int DR, bool R12, int SUMS>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1)))
+void k_md_site_xcd(const vec2_t<S>* __restrict__ U_in,
                                                         vec2_t<S>* __restrict__ U_out,
                                                         S* __restrict__ E, Geom g, T fscale, T kick,
-                                                        T dfac, int64_t slab_p, int64_t slab_g)
+                                                        T dfac, int64_t slab_p, int64_t slab_g,
+                                                        double* __restrict__ partials, vec2_t<S>* row2)
 {
     int64_t logical = xcd_remap(blockIdx.x, gridDim.x);
     if (slab_p > 0) logical = slab_block(logical, slab_p, slab_g);
     const int64_t s = logical * kBlock + threadIdx.x;
-    if (s >= g.nsite) return;
     // f32 two-row reads keep the own links in LDS (72 KiB at D = 4: two
     // workgroups per CU, the occupancy the registers allow anyway)
     constexpr bool OWN = R12 && sizeof(T) == 4;
     __shared__ vec2_t<T> own_lds[OWN ? D * 9 * kBlock : 1];
-    md_site<D, S, T, DR, R12, OWN, R12 && DR == DRIFT_EXP>(U_in, U_out, E, g, s, fscale, kick, dfac, own_lds);
+    constexpr bool R01 = R12 && DR == DRIFT_EXP;
+    if constexpr (SUMS == SUMS_NONE) {
+        if (s >= g.nsite) return;
+        md_site<D, S, T, DR, R12, OWN, R01>(U_in, U_out, E, g, s, fscale, kick, dfac, own_lds, nullptr, row2);
+    } else {
+        MdAcc<T> acc;
+        if (s < g.nsite)
+            md_site<D, S, T, DR, R12, OWN, R01, SUMS>(U_in, U_out, E, g, s, fscale, kick, dfac, own_lds, &acc,
+                                                      row2);
+        double v[2] = {(double)acc.tr, (double)acc.e2};
+        block_reduce_to<2>(v, partials + (int64_t)blockIdx.x * kReduceSlots);
+    }
 }
 
 // VAR_SITE_TILE: per site as VAR_SITE_SLAB (same block order), two-row
 // links only: the workgroup first stages rows 0, 1 of the links of its tile
 // and of the two neighbouring x_1 rows in LDS (TileShape), then reads every
 // link reachable by steps in x_0 and x_1 from there and the rest from U_in.
-template <int D, typename S, typename T, int DR>
-__global__ __launch_bounds__(kBlock) void k_md_site_tile(const vec2_t<S>* __restrict__ U_in,
+template <int D, typename S, typename T, int DR, int SUMS>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2)))
+void k_md_site_tile(const vec2_t<S>* __restrict__ U_in,
                                                          vec2_t<S>* __restrict__ U_out,
                                                          S* __restrict__ E, Geom g, T fscale, T kick,
-                                                         T dfac, int64_t slab_p, int64_t slab_g)
+                                                         T dfac, int64_t slab_p, int64_t slab_g,
+                                                         double* __restrict__ partials, vec2_t<S>* row2)
 {
     extern __shared__ unsigned char tile_raw[];
     auto* tile = reinterpret_cast<vec2_t<T>*>(tile_raw);
@@ -180,6 +197,7 @@ __global__ __launch_bounds__(kBlock) void k_md_site_tile(const vec2_t<S>* __rest
     __syncthreads();
     const Neighbours<D> nb = neighbours<D>(g, s);
     const int own = tile_slot(ts, r, c, 0, 0);
+    MdAcc<T> acc;
     T f[D][8];
     {
         mat3<T> sig[D];
@@ -190,15 +208,22 @@ __global__ __launch_bounds__(kBlock) void k_md_site_tile(const vec2_t<S>* __rest
             mat3<T> m;
             mul<false, false>(u, sig[mu], m);
             im_trace_generators(m, f[mu]);
+            if constexpr (SUMS != SUMS_NONE) acc.tr += re_trace(m);
         }
     }
     T e[D][8];
 #pragma unroll
     for (int mu = 0; mu < D; ++mu) {
         load_e<T, S, kCacheStream>(E, g.npad, mu, s, e[mu]);
+        if constexpr (SUMS == SUMS_BEFORE_KICK) acc.e2 += e_squares8<T>(e[mu]);
 #pragma unroll
         for (int a = 0; a < 8; ++a) e[mu][a] = fma(kick, fscale * f[mu][a], e[mu][a]);
+        if constexpr (SUMS == SUMS_AFTER_KICK) acc.e2 += e_squares8<T>(e[mu]);
         store_e<T, S, kCacheStream>(E, g.npad, mu, s, e[mu]);
+    }
+    if (row2) {
+#pragma unroll
+        for (int mu = 0; mu < D; ++mu) store_link_row2<T, S>(row2, g.npad, mu, s, tile_r12<T>(tile, ts, mu, own));
     }
     if constexpr (DR >= 0) {
 #pragma unroll
@@ -206,6 +231,10 @@ __global__ __launch_bounds__(kBlock) void k_md_site_tile(const vec2_t<S>* __rest
             const mat3<T> u = tile_r12<T>(tile, ts, mu, own);
             drift_store<DR, DR == DRIFT_EXP, S, T>(u, e[mu], dfac, U_out, g, mu, s);
         }
+    }
+    if constexpr (SUMS != SUMS_NONE) {
+        double v[2] = {(double)acc.tr, (double)acc.e2};
+        block_reduce_to<2>(v, partials + (int64_t)blockIdx.x * kReduceSlots);
     }
 }
 
@@ -251,9 +280,30 @@ static dim3 link_grid(const lqcd_lattice* L)
     return dim3(grid_for(L->g.nsite), (unsigned)L->g.dim);
 }
 
+// the per-site kernels with the Hamiltonian sums of the launch (x->sums: 0,
+// or SUMS_BEFORE_KICK for a kick+drift, SUMS_AFTER_KICK for a kick) and the
+// row-2 write (x->row2)
+template <int D, typename S, int DR, bool R12, int SUMS>
+static void launch_site(const lqcd_lattice* L, bool tile, unsigned nsb, const vec2_t<S>* ui, vec2_t<S>* uo, S* e,
+                        S fs, S k, S df, int64_t slab_p, int64_t slab_g, const MdExtra* x)
+{
+    hipStream_t st = L->ctx->stream;
+    double* partials = x ? x->partials : nullptr;
+    auto* row2 = (vec2_t<S>*)(x ? x->row2 : nullptr);
+    if constexpr (R12 && sizeof(S) == 4) {
+        if (tile) {
+            k_md_site_tile<D, S, S, DR, SUMS><<<nsb, kBlock, md_tile_bytes(L), st>>>(ui, uo, e, L->g, fs, k, df, slab_p,
+                                                                                    slab_g, partials, row2);
+            return;
+        }
+    }
+    k_md_site_xcd<D, S, S, DR, R12, SUMS><<<nsb, kBlock, 0, st>>>(ui, uo, e, L->g, fs, k, df, slab_p, slab_g,
+                                                                  partials, row2);
+}
+
 template <int D, typename S, int DR, bool R12>
 static void launch_md_variant_r(const lqcd_lattice* L, int variant, const void* U_in, void* U_out, void* E,
-                              double kick, double dfac)
+                                double kick, double dfac, const MdExtra* x)
 {
     const S fs = (S)md_force_scale(L->size), k = (S)kick, df = (S)dfac;
     const auto* ui = (const vec2_t<S>*)U_in;
@@ -261,25 +311,12 @@ static void launch_md_variant_r(const lqcd_lattice* L, int variant, const void* 
     auto* e = (S*)E;
     hipStream_t st = L->ctx->stream;
     const unsigned nsb = grid_for(L->g.nsite);
+    const int sums = x ? x->sums : SUMS_NONE;
     switch (variant) {
     case VAR_SITE_TILE:
-        if constexpr (R12 && sizeof(S) == 4) {
-            if (md_tile_fits(L)) {
-                int64_t slab_p = 0, slab_g = 0;
-                const int64_t Ls = L->g.extent[D - 1];
-                const int64_t slice = L->g.nsite / Ls;
-                if (slice % kBlock == 0 && Ls % 8 == 0) {
-                    slab_p = slice / kBlock;
-                    slab_g = Ls / 8;
-                }
-                k_md_site_tile<D, S, S, DR><<<nsb, kBlock, md_tile_bytes(L), st>>>(ui, uo, e, L->g, fs, k, df,
-                                                                                    slab_p, slab_g);
-                break;
-            }
-        }
-        [[fallthrough]];
     case VAR_SITE_XCD:
     case VAR_SITE_SLAB: {
+        const bool tile = variant == VAR_SITE_TILE && R12 && sizeof(S) == 4 && md_tile_fits(L);
         int64_t slab_p = 0, slab_g = 0;
         const int64_t Ls = L->g.extent[D - 1];
         const int64_t slice = L->g.nsite / Ls;
@@ -287,7 +324,11 @@ static void launch_md_variant_r(const lqcd_lattice* L, int variant, const void* 
             slab_p = slice / kBlock;
             slab_g = Ls / 8;
         }
-        k_md_site_xcd<D, S, S, DR, R12><<<nsb, kBlock, 0, st>>>(ui, uo, e, L->g, fs, k, df, slab_p, slab_g);
+        constexpr int SUMS_ON = DR >= 0 ? SUMS_BEFORE_KICK : SUMS_AFTER_KICK;
+        if (sums == SUMS_NONE)
+            launch_site<D, S, DR, R12, SUMS_NONE>(L, tile, nsb, ui, uo, e, fs, k, df, slab_p, slab_g, x);
+        else
+            launch_site<D, S, DR, R12, SUMS_ON>(L, tile, nsb, ui, uo, e, fs, k, df, slab_p, slab_g, x);
         break;
     }
     case VAR_LINK_XCD:
@@ -303,15 +344,15 @@ static void launch_md_variant_r(const lqcd_lattice* L, int variant, const void* 
 // has checked them to be in SU(3) (lqcd_lattice::md_r12)
 template <int D, typename S, int DR>
 static void launch_md_variant(const lqcd_lattice* L, int variant, const void* U_in, void* U_out, void* E,
-                              double kick, double dfac)
+                              double kick, double dfac, const MdExtra* x)
 {
     if constexpr (DR != DRIFT_ADDITIVE) {
         if (L->md_r12) {
-            launch_md_variant_r<D, S, DR, true>(L, variant, U_in, U_out, E, kick, dfac);
+            launch_md_variant_r<D, S, DR, true>(L, variant, U_in, U_out, E, kick, dfac, x);
             return;
         }
     }
-    launch_md_variant_r<D, S, DR, false>(L, variant, U_in, U_out, E, kick, dfac);
+    launch_md_variant_r<D, S, DR, false>(L, variant, U_in, U_out, E, kick, dfac, x);
 }
 
 static int chosen_variant(const lqcd_lattice* L)
@@ -336,42 +377,48 @@ hipError_t launch_force_store(const lqcd_lattice* L, const void* U, double* d_ou
     return hipGetLastError();
 }
 
-hipError_t launch_kick_variant(const lqcd_lattice* L, int variant, const void* U, void* E, double kick)
+hipError_t launch_kick_variant(const lqcd_lattice* L, int variant, const void* U, void* E, double kick,
+                               const MdExtra* x)
 {
-    LQCD_DISPATCH_DS(L, launch_md_variant<D, S, -1>(L, variant, U, nullptr, E, kick, 0.0));
+    LQCD_DISPATCH_DS(L, launch_md_variant<D, S, -1>(L, variant, U, nullptr, E, kick, 0.0, x));
     return hipGetLastError();
 }
 
 hipError_t launch_kick_drift_variant(const lqcd_lattice* L, int variant, const void* U_in, void* U_out,
-                                     void* E, double kick, double dt, int drift_mode)
+                                     void* E, double kick, double dt, int drift_mode, const MdExtra* x)
 {
     const double df = md_drift_scale(dt, L->size);
     if (drift_mode == DRIFT_EXP) {
-        LQCD_DISPATCH_DS(L, launch_md_variant<D, S, DRIFT_EXP>(L, variant, U_in, U_out, E, kick, df));
+        LQCD_DISPATCH_DS(L, launch_md_variant<D, S, DRIFT_EXP>(L, variant, U_in, U_out, E, kick, df, x));
     } else {
-        LQCD_DISPATCH_DS(L, launch_md_variant<D, S, DRIFT_ADDITIVE>(L, variant, U_in, U_out, E, kick, df));
+        LQCD_DISPATCH_DS(L, launch_md_variant<D, S, DRIFT_ADDITIVE>(L, variant, U_in, U_out, E, kick, df, x));
     }
     return hipGetLastError();
 }
 
-hipError_t launch_kick(const lqcd_lattice* L, const void* U, void* E, double kick)
+hipError_t launch_kick(const lqcd_lattice* L, const void* U, void* E, double kick, const MdExtra* x)
 {
     lqcd_prof_event ev;
     prof_begin(L->ctx, PROF_KICK, &ev);
-    const hipError_t e = launch_kick_variant(L, chosen_variant(L), U, E, kick);
+    const hipError_t e = launch_kick_variant(L, chosen_variant(L), U, E, kick, x);
     prof_end(L->ctx, &ev);
     return e;
 }
 
 hipError_t launch_kick_drift(const lqcd_lattice* L, const void* U_in, void* U_out, void* E, double kick,
-                             double dt, int drift_mode)
+                             double dt, int drift_mode, const MdExtra* x)
 {
     lqcd_prof_event ev;
     prof_begin(L->ctx, PROF_KICK_DRIFT, &ev);
     const hipError_t e =
-        launch_kick_drift_variant(L, chosen_variant(L), U_in, U_out, E, kick, dt, drift_mode);
+        launch_kick_drift_variant(L, chosen_variant(L), U_in, U_out, E, kick, dt, drift_mode, x);
     prof_end(L->ctx, &ev);
     return e;
+}
+
+bool md_variant_has_sums(int variant)
+{
+    return variant == VAR_SITE_XCD || variant == VAR_SITE_SLAB || variant == VAR_SITE_TILE;
 }
 
 hipError_t launch_drift(const lqcd_lattice* L, const void* U_in, void* U_out, const void* E, double dt,

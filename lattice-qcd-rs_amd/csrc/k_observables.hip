@@ -180,6 +180,12 @@ hipError_t launch_mh_stats(const lqcd_lattice* L, double* d_out2)
     return hipGetLastError();
 }
 
+hipError_t launch_final_reduce_at(const lqcd_lattice* L, const double* partials, int nblocks, double* d_out2)
+{
+    k_final_reduce<2><<<1, kBlock, 0, L->ctx->stream>>>(partials, nblocks, d_out2);
+    return hipGetLastError();
+}
+
 hipError_t launch_final_reduce1(const lqcd_lattice* L, int nblocks, double* d_out1)
 {
     k_final_reduce<1><<<1, kBlock, 0, L->ctx->stream>>>(L->partials, nblocks, d_out1);

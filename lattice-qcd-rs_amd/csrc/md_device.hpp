@@ -380,11 +380,44 @@ __device__ __forceinline__ void md_link(const vec2_t<S>* __restrict__ U_in, vec2
     finish_link<DR, S, T, R01>(u, sigma, U_out, E, g, mu, s, fscale, kick, dfac);
 }
 
-// all D links of site s, sharing the staple loads
-template <int D, typename S, typename T, int DR, bool R12 = false, bool OWN = false, bool R01 = false>
+// The Hamiltonian sums a fused MD launch can produce on the way (SUMS): the
+// trace sum sum_mu Re Tr(U_mu(x) Sigma_mu(x)) of the links it reads (every
+// plaquette enters it through each of its 4 links, so it is 4 sum_P Re Tr P)
+// and the momentum sum sum_mu,a e^a_mu(x)^2 of E before the kick
+// (SUMS_BEFORE_KICK: the first launch of a trajectory, H_old) or after it
+// (SUMS_AFTER_KICK: the final kick, H_new); accumulated per thread in double.
+enum MdSums { SUMS_NONE = 0, SUMS_BEFORE_KICK = 1, SUMS_AFTER_KICK = 2 };
+template <typename T>
+struct MdAcc {
+    T tr = T(0);
+    T e2 = T(0);
+};
+
+template <typename T>
+__device__ __forceinline__ T re_trace(const mat3<T>& m)
+{
+    return m.e[0].re + m.e[4].re + m.e[8].re;
+}
+
+template <typename T>
+__device__ __forceinline__ T e_squares8(const T* e)
+{
+    T q = T(0);
+#pragma unroll
+    for (int a = 0; a < 8; ++a) q = fma(e[a], e[a], q);
+    return q;
+}
+
+// all D links of site s, sharing the staple loads.  SUMS: add to acc (see
+// MdSums).  row2 (two-row reads only): row 2 of the site's links, completed
+// here anyway, is written to row2 (the buffer the links were read from, so a
+// final kick leaves complete links behind).
+template <int D, typename S, typename T, int DR, bool R12 = false, bool OWN = false, bool R01 = false,
+          int SUMS = SUMS_NONE>
 __device__ __forceinline__ void md_site(const vec2_t<S>* __restrict__ U_in, vec2_t<S>* __restrict__ U_out,
                                         S* __restrict__ E, const Geom& g, int64_t s, T fscale, T kick,
-                                        T dfac, vec2_t<T>* own = nullptr)
+                                        T dfac, vec2_t<T>* own = nullptr, MdAcc<T>* acc = nullptr,
+                                        vec2_t<S>* row2 = nullptr)
 {
     const Neighbours<D> nb = neighbours<D>(g, s);
     // OWN (two-row reads, f32): the own links into the thread's LDS slots
@@ -400,9 +433,11 @@ __device__ __forceinline__ void md_site(const vec2_t<S>* __restrict__ U_in, vec2
 #pragma unroll
         for (int mu = 0; mu < D; ++mu) {
             const mat3<T> u = OWN ? own_get<T>(own, mu) : load_link_sel<R12, T, S>(U_in, g.npad, mu, s);
+
             mat3<T> m;
             mul<false, false>(u, sig[mu], m);
             im_trace_generators(m, f[mu]);
+            if constexpr (SUMS != SUMS_NONE) acc->tr += re_trace(m);
         }
     }
     // phase 2: momentum kicks of the D links
@@ -410,9 +445,20 @@ __device__ __forceinline__ void md_site(const vec2_t<S>* __restrict__ U_in, vec2
 #pragma unroll
     for (int mu = 0; mu < D; ++mu) {
         load_e<T, S, kCacheStream>(E, g.npad, mu, s, e[mu]);
+        if constexpr (SUMS == SUMS_BEFORE_KICK) acc->e2 += e_squares8<T>(e[mu]);
 #pragma unroll
         for (int a = 0; a < 8; ++a) e[mu][a] = fma(kick, fscale * f[mu][a], e[mu][a]);
+        if constexpr (SUMS == SUMS_AFTER_KICK) acc->e2 += e_squares8<T>(e[mu]);
         store_e<T, S, kCacheStream>(E, g.npad, mu, s, e[mu]);
+    }
+    if constexpr (R12) {
+        if (row2) {
+#pragma unroll
+            for (int mu = 0; mu < D; ++mu) {
+                const mat3<T> u = OWN ? own_get<T>(own, mu) : load_link_r12<T, S>(U_in, g.npad, mu, s);
+                store_link_row2<T, S>(row2, g.npad, mu, s, u);
+            }
+        }
     }
     // phase 3: drifts (U_mu(x) is read again, from cache)
     if constexpr (DR >= 0) {

@@ -102,11 +102,14 @@ def test_batch_hmc_matches_single_lattices(precision, config, drift, scheme, d):
         for t in range(n_traj):
             s = stats[t, i]
             r = ref[t]
-            tol_h = 1e-12 * abs(r.h_old) + 1e-12
+            # a lattice of its own sums H inside its MD kernels (f32 traces
+            # for f32 storage), the batch with its plaquette kernel
+            tol_h = (1e-12 if precision == lqcd.F64 else 1e-8) * abs(r.h_old) + 1e-12
             assert abs(s["h_old"] - r.h_old) <= tol_h
             assert abs(s["h_new"] - r.h_new) <= tol_h
             assert s["accepted"] == r.accepted
-            assert abs(s["probability"] - r.probability) <= 1e-9
+            # p = exp(-dH) moves with dH: 1e-4 covers the f32 traces' 2e-5 in dH
+            assert abs(s["probability"] - r.probability) <= (1e-9 if precision == lqcd.F64 else 1e-4)
     assert np.all(np.isfinite(stats["delta_h"]))
 
 
