@@ -6,7 +6,9 @@ Workload (one "step" = one HMC trajectory, everything resident in HBM):
   leapfrog steps of dt = 0.025 with the su(3)-exponential link update, links
   and E field stored in FP32, FP64 for the Hamiltonian / plaquette sums, the
   accept decision and the reunitarisation that follows every trajectory.
-  Momenta N(0, 0.5/beta) are drawn on the device each trajectory.
+  Momenta N(0, 0.5/beta) are drawn on the device each trajectory.  H_old and
+  H_new are summed inside the first MD launch and the final half kick of each
+  trajectory (per-link traces in the storage precision, sums in FP64).
 
 value = link updates/s = links * MD steps * trajectories / wall time.
 The same trajectories are run with FP64 storage for the traj/s comparison at
