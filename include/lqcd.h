@@ -18,7 +18,11 @@
  *      efield [site][mu][8] doubles (src/field.rs:1025-1027);
  *      site   = sum_i x_i * prod_{k<i} L_k (src/lattice.rs:909-915).
  *  - Device buffers are owned by the library; fields stay resident in HBM
- *    between calls.  Calls are synchronous unless stated otherwise.
+ *    between calls.  Calls are synchronous unless stated otherwise; the
+ *    ones that only enqueue device work (lqcd_efield_random,
+ *    lqcd_normalize_link_matrices) return once it is enqueued on the
+ *    context's stream: every later call on the context sees their result,
+ *    and lqcd_ctx_synchronize waits for them.
  *  - A context owns one HIP stream and is not thread-safe: use one context
  *    per host thread.
  */
@@ -109,7 +113,7 @@ int lqcd_efield_cold(lqcd_lattice* lat);
 /* Momentum refresh drawn on the device (replaces EField::new_determinist with
  * Normal(0, sigma), src/field.rs:1086-1099; the reference uses
  * sigma = 0.5 / beta, src/simulation/state.rs:1097).  Counter-based stream:
- * Philox4x32-10 keyed by seed, counter selects the draw. */
+ * Philox4x32-10 keyed by seed, counter selects the draw.  Enqueued only. */
 int lqcd_efield_random(lqcd_lattice* lat, uint64_t seed, uint64_t counter, double sigma);
 
 /* EField::gauss (src/field.rs:1174-1195) at every site:
@@ -322,7 +326,8 @@ int lqcd_state_read_host(const char* path, int format, int with_efield, int dim,
 int lqcd_state_read(lqcd_lattice* lat, const char* path, int format, int with_efield, double* beta,
                     uint64_t* t);
 
-/* LatticeStateDefault::normalize_link_matrices (src/simulation/state.rs:754-756) */
+/* LatticeStateDefault::normalize_link_matrices (src/simulation/state.rs:754-756).
+ * Enqueued only. */
 int lqcd_normalize_link_matrices(lqcd_lattice* lat);
 
 /* ------------------------------------------------------------- tuning */

@@ -621,8 +621,8 @@ int lqcd_efield_random(lqcd_lattice* L, uint64_t seed, uint64_t counter, double 
     CHECK_ARG(L, "lattice is null");
     CHECK_ARG(sigma >= 0.0 && std::isfinite(sigma), "sigma must be finite and >= 0");
     DeviceGuard dg(L->ctx->device);
+    // enqueued only: later calls on the context are stream-ordered after it
     HIP_TRY(launch_device_momenta(L, L->E, seed, counter, sigma));
-    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
     return LQCD_OK;
 }
 
@@ -1196,8 +1196,8 @@ int lqcd_normalize_link_matrices(lqcd_lattice* L)
 {
     CHECK_ARG(L, "lattice is null");
     DeviceGuard dg(L->ctx->device);
+    // enqueued only: later calls on the context are stream-ordered after it
     HIP_TRY(launch_reunitarize(L, L->U[L->cur]));
-    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
     L->su3_known = 0;
     return LQCD_OK;
 }
