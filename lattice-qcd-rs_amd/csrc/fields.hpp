@@ -73,7 +73,12 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off
     } else {
         static_assert(sizeof(E) == 16, "element of 4, 8 or 16 bytes");
         using V = __attribute__((ext_vector_type(4))) uint32_t;
-        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V, v), r, (int)off, (int)soff, CP);
+        // soffset 0, the plane offset in the VGPR offset: the compiler inserts
+        // the wait state that a VALU write of the data VGPRs right after a
+        // store of more than 64 bits needs only when soffset is not a
+        // register, and without it such writes reach memory in place of the
+        // stored data
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V, v), r, (int)(off + soff), 0, CP);
     }
 }
 
