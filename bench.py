@@ -693,7 +693,7 @@ def bench_fields(args, lqcd):
                "sample": f"{k} F_mu_nu fields of the oracle on 16^4, 16 OpenMP threads, {t:.1f} s"}
     return contract_line(
         args, "F_mu_nu sites/s, 32^4", nsite * args.steps / seconds, "sites/s", args.steps, seconds, "f64",
-        cfg, roofline_from(rate, "k_clover_fields<4,float,1> (clover, F_mu_nu)"), cpu)
+        cfg, roofline_from(rate, "k_f_mu_nu_blocks<4,float> (clover, F_mu_nu, one thread per site and pair)"), cpu)
 
 
 def bench_sweeps(args, lqcd):

@@ -4,7 +4,8 @@
 //
 // The output is in the host layout of the reference (18 doubles per matrix,
 // column-major 3x3 complex): [site][i][j] for the clover and F (stores
-// staged through LDS), [site][dir] for B; one thread per site.
+// staged through LDS), [site][dir] for B.  F takes one thread per (site,
+// pair i < j) (k_f_mu_nu_blocks); the clover and B one thread per site.
 #include "dispatch.hpp"
 #include "fields.hpp"
 
@@ -79,25 +80,57 @@ __device__ __forceinline__ void clover_at(const vec2_t<S>* __restrict__ U, const
     for (int k = 0; k < 9; ++k) c.e[k] = {c.e[k].re + p.e[k].re, c.e[k].im + p.e[k].im};
 }
 
+// Link loaders: U_d(y) through a resource over the 9 planes of direction d
+// (uniform d), or through one resource over all 9 D planes (d may differ
+// between the lanes of a wave; the planes must span less than 4 GiB).
+template <typename S>
+struct DirLinks {
+    const vec2_t<S>* __restrict__ U;
+    int64_t npad;
+    __device__ __forceinline__ mat3<double> operator()(int d, int64_t y) const
+    {
+        return load_link<double, S>(U, npad, d, y);
+    }
+};
+
+template <typename S, int D>
+struct AllLinks {
+    const vec2_t<S>* __restrict__ U;
+    int64_t npad;
+    __device__ __forceinline__ mat3<double> operator()(int d, int64_t y) const
+    {
+        using E = vec2_t<S>;
+        const __amdgpu_buffer_rsrc_t r = planes_rsrc(U, npad, 9 * D);
+        const uint32_t plane = (uint32_t)npad * (uint32_t)sizeof(E);
+        const uint32_t off = ((uint32_t)d * 9u * (uint32_t)npad + (uint32_t)y) * (uint32_t)sizeof(E);
+        mat3<double> m;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const E v = buf_load<E>(r, off, (uint32_t)k * plane);
+            m.e[k] = {double(v.x), double(v.y)};
+        }
+        return m;
+    }
+};
+
 // One leaf of the clover in the form of pij: S = op(ub) op(uab) op(uba)^+,
 // P = op(ua) S^+, where op(.) is the adjoint when the template flag is set
 // and each matrix is the link U_d(site) named by its arguments.
-template <typename S, bool DUB, bool DUAB, bool DUBA, bool DUA>
-__device__ __forceinline__ void leaf(const vec2_t<S>* __restrict__ U, const Geom& g, int db, int64_t sb,
-                                     int dab, int64_t sab, int dba, int64_t sba, int da, int64_t sa,
-                                     mat3<double>& p)
+template <typename LD, bool DUB, bool DUAB, bool DUBA, bool DUA>
+__device__ __forceinline__ void leaf(const LD& ld, int db, int64_t sb, int dab, int64_t sab, int dba, int64_t sba,
+                                     int da, int64_t sa, mat3<double>& p)
 {
     mat3<double> t, st;
     {
-        const mat3<double> ub = load_link<double, S>(U, g.npad, db, sb);
-        const mat3<double> uab = load_link<double, S>(U, g.npad, dab, sab);
+        const mat3<double> ub = ld(db, sb);
+        const mat3<double> uab = ld(dab, sab);
         mul<DUB, DUAB>(ub, uab, t);
     }
     {
-        const mat3<double> uba = load_link<double, S>(U, g.npad, dba, sba);
+        const mat3<double> uba = ld(dba, sba);
         mul<false, !DUBA>(t, uba, st);
     }
-    const mat3<double> ua = load_link<double, S>(U, g.npad, da, sa);
+    const mat3<double> ua = ld(da, sa);
     mul<DUA, true>(ua, st, p);
 }
 
@@ -109,26 +142,36 @@ __device__ __forceinline__ void leaf(const vec2_t<S>* __restrict__ U, const Geom
 //   P(j,-i):  ub = U_i(x-i)^+,  uab = U_j(x-i),      uba = U_i(x+j-i)^+,  ua = U_j(x)
 //   P(-i,-j): ub = U_j(x-j)^+,  uab = U_i(x-j-i)^+,  uba = U_j(x-i-j)^+,  ua = U_i(x-i)^+
 //   P(-j,i):  ub = U_i(x),      uab = U_j(x+i-j)^+,  uba = U_i(x-j),      ua = U_j(x-j)^+
-template <int D, typename S>
-__device__ __forceinline__ void clover_offdiag(const vec2_t<S>* __restrict__ U, const Geom& g, int64_t s,
-                                               const Neighbours<D>& nb, int i, int j, mat3<double>& c)
+template <int D, typename LD>
+__device__ __forceinline__ void clover_offdiag(const LD& ld, int64_t s, const Neighbours<D>& nb, int i, int j,
+                                               mat3<double>& c)
 {
-    const int64_t pi = s + nb.up[i], pj = s + nb.up[j], mi = s + nb.dn[i], mj = s + nb.dn[j];
-    const int64_t pj_mi = pj + nb.dn[i], mi_mj = mi + nb.dn[j], pi_mj = pi + nb.dn[j];
+    // the offsets for directions i, j by selects (i, j may differ between
+    // lanes: an indexed read of nb would go through scratch memory)
+    int32_t ui = 0, uj = 0, di = 0, dj = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        ui = d == i ? nb.up[d] : ui;
+        uj = d == j ? nb.up[d] : uj;
+        di = d == i ? nb.dn[d] : di;
+        dj = d == j ? nb.dn[d] : dj;
+    }
+    const int64_t pi = s + ui, pj = s + uj, mi = s + di, mj = s + dj;
+    const int64_t pj_mi = pj + di, mi_mj = mi + dj, pi_mj = pi + dj;
     // one leaf at a time: without the scheduling fences the loads of all
     // four leaves are issued up front and the kernel needs 250 VGPRs
     mat3<double> p;
-    leaf<S, false, false, false, false>(U, g, j, s, i, pj, j, pi, i, s, c);
+    leaf<LD, false, false, false, false>(ld, j, s, i, pj, j, pi, i, s, c);
     __builtin_amdgcn_sched_barrier(0);
-    leaf<S, true, false, true, false>(U, g, i, mi, j, mi, i, pj_mi, j, s, p);
+    leaf<LD, true, false, true, false>(ld, i, mi, j, mi, i, pj_mi, j, s, p);
 #pragma unroll
     for (int k = 0; k < 9; ++k) c.e[k] = {c.e[k].re + p.e[k].re, c.e[k].im + p.e[k].im};
     __builtin_amdgcn_sched_barrier(0);
-    leaf<S, true, true, true, true>(U, g, j, mj, i, mi_mj, j, mi_mj, i, mi, p);
+    leaf<LD, true, true, true, true>(ld, j, mj, i, mi_mj, j, mi_mj, i, mi, p);
 #pragma unroll
     for (int k = 0; k < 9; ++k) c.e[k] = {c.e[k].re + p.e[k].re, c.e[k].im + p.e[k].im};
     __builtin_amdgcn_sched_barrier(0);
-    leaf<S, false, true, false, true>(U, g, i, s, j, pi_mj, i, mj, j, mj, p);
+    leaf<LD, false, true, false, true>(ld, i, s, j, pi_mj, i, mj, j, mj, p);
 #pragma unroll
     for (int k = 0; k < 9; ++k) c.e[k] = {c.e[k].re + p.e[k].re, c.e[k].im + p.e[k].im};
 }
@@ -204,7 +247,7 @@ __global__ __launch_bounds__(kBlock) void k_magnetic_field(const vec2_t<S>* __re
     for (int i = 0; i < D; ++i)
         for (int j = i + 1; j < D; ++j) {
             mat3<double> c, ca, f;
-            clover_offdiag<D, S>(U, g, s, nb, i, j, c);
+            clover_offdiag<D>(DirLinks<S>{U, g.npad}, s, nb, i, j, c);
             adjoint(c, ca);
 #pragma unroll
             for (int k = 0; k < 9; ++k) f.e[k] = {(c.e[k].re - ca.e[k].re) * inv, (c.e[k].im - ca.e[k].im) * inv};
@@ -245,7 +288,7 @@ __global__ __launch_bounds__(kBlock) void k_clover_fields(const vec2_t<S>* __res
         store_slot_staged(stage, out, s0, g.nsite, NS, i * D + i, dg);
         for (int j = i + 1; j < D; ++j) {
             mat3<double> c = zero, ca;
-            if (active) clover_offdiag<D, S>(U, g, s, nb, i, j, c);
+            if (active) clover_offdiag<D>(DirLinks<S>{U, g.npad}, s, nb, i, j, c);
             adjoint(c, ca);
             if (MODE == 1) {
 #pragma unroll
@@ -258,6 +301,106 @@ __global__ __launch_bounds__(kBlock) void k_clover_fields(const vec2_t<S>* __res
             store_slot_staged(stage, out, s0, g.nsite, NS, i * D + j, c);
             store_slot_staged(stage, out, s0, g.nsite, NS, j * D + i, ca);
         }
+    }
+}
+
+// F_mu_nu of every site (LinkMatrix::f_mu_nu, field.rs:822-834) with the
+// output written in whole 128-byte lines.  A workgroup owns 32 consecutive
+// sites and has one thread per (site, pair i < j): 32 D (D - 1) / 2 threads,
+// the pair fixed per half wavefront (so a load instruction reads 32
+// consecutive sites; the link direction differs between the halves, hence the
+// resource over all planes).  Each thread forms F_ij = (C_ij - C_ij^+) / (8 a^2)
+// and writes F_ij and F_ji = -F_ij into an LDS image of the sites' output
+// (site-major, the host layout); the diagonal slots are zero.  The image is
+// written out in two halves of the D^2 slots (8 of 16 slots, 1,152 B = nine
+// lines per site for D = 4), each a run of whole lines per site.  (One row of
+// x_0 per workgroup in block order: handing XCDs contiguous block ranges was
+// slower here.)
+// the output is written once and read by the host or a later kernel: its
+// stores are marked non-temporal, so that the L2 keeps the links
+using d2v = double __attribute__((ext_vector_type(2)));
+
+template <int D>
+struct FPairs {
+    static constexpr int n = D * (D - 1) / 2;
+    static constexpr int threads = 32 * n;
+    static constexpr int slots = D * D;
+    static constexpr int half0 = slots / 2;                 // slots [0, half0), then [half0, slots)
+    static constexpr int stage_slots = slots - half0;      // the larger half
+    static constexpr int site_doubles = stage_slots * 18 + 2;  // padded: lanes of one half wave spread over banks
+};
+
+template <int D>
+__device__ __forceinline__ void pair_of(int p, int& i, int& j)
+{
+    i = 0;
+    int k = p;
+    while (k >= D - 1 - i) {
+        k -= D - 1 - i;
+        ++i;
+    }
+    j = i + 1 + k;
+}
+
+template <int D, typename S>
+__global__ __launch_bounds__(FPairs<D>::threads) void k_f_mu_nu_blocks(const vec2_t<S>* __restrict__ U, Geom g,
+                                                                      double* __restrict__ out, double inv)
+{
+    using P = FPairs<D>;
+    __shared__ double stage[32 * P::site_doubles];
+    const int t = threadIdx.x, pr = t >> 5, sl = t & 31;
+    const int64_t s0 = (int64_t)blockIdx.x * 32;
+    const int64_t s = s0 + sl;
+    const int nsl = g.nsite - s0 < 32 ? (int)(g.nsite - s0) : 32;
+    int i, j;
+    pair_of<D>(pr, i, j);
+    mat3<double> f = {};
+    if (sl < nsl) {
+        const Neighbours<D> nb = neighbours<D>(g, s);
+        mat3<double> c;
+        clover_offdiag<D>(AllLinks<S, D>{U, g.npad}, s, nb, i, j, c);
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const cplx<double> a = c.e[r * 3 + q], b = c.e[q * 3 + r];
+                f.e[r * 3 + q] = {(a.re - b.re) * inv, (a.im + b.im) * inv};
+            }
+    }
+    double* const mine = stage + sl * P::site_doubles;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int lo = h == 0 ? 0 : P::half0, hi = h == 0 ? P::half0 : P::slots;
+        // F_ij, F_ji = -F_ij and the diagonal zeros of this half
+        auto put = [&](int slot, double sign) __attribute__((always_inline)) {
+            if (slot < lo || slot >= hi) return;
+            double* m = mine + (slot - lo) * 18;
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    m[2 * (q * 3 + r)] = sign * f.e[r * 3 + q].re;
+                    m[2 * (q * 3 + r) + 1] = sign * f.e[r * 3 + q].im;
+                }
+        };
+        put(i * D + j, 1.0);
+        put(j * D + i, -1.0);
+        for (int d = pr; d < D; d += P::n) {
+            const int slot = d * D + d;
+            if (slot >= lo && slot < hi)
+#pragma unroll
+                for (int k = 0; k < 18; ++k) mine[(slot - lo) * 18 + k] = 0.0;
+        }
+        __syncthreads();
+        // the sites' runs of (hi - lo) matrices, 9 double2 each
+        const int per_site = (hi - lo) * 9;
+        const int total = nsl * per_site;
+        for (int q = t; q < total; q += P::threads) {
+            const int site = q / per_site, r = q - site * per_site;
+            const double2 v = *reinterpret_cast<const double2*>(stage + site * P::site_doubles + 2 * r);
+            __builtin_nontemporal_store(d2v{v.x, v.y}, reinterpret_cast<d2v*>(out + ((s0 + site) * P::slots + lo) * 18) + r);
+        }
+        __syncthreads();
     }
 }
 
@@ -306,8 +449,14 @@ hipError_t launch_clover_fields(const lqcd_lattice* L, int mode, double* d_out)
                                 (const vec2_t<S>*)L->U[L->cur], L->g, d_out, inv));
         break;
     case 1:
-        LQCD_DISPATCH_DS(L, k_clover_fields<D, S, 1><<<grid, kBlock, 0, L->ctx->stream>>>(
-                                (const vec2_t<S>*)L->U[L->cur], L->g, d_out, inv));
+        if ((uint64_t)L->g.npad * 9 * L->g.dim * L->elem <= kMaxPlaneBytes) {
+            const unsigned nb = (unsigned)((L->g.nsite + 31) / 32);
+            LQCD_DISPATCH_DS(L, k_f_mu_nu_blocks<D, S><<<nb, FPairs<D>::threads, 0, L->ctx->stream>>>(
+                                    (const vec2_t<S>*)L->U[L->cur], L->g, d_out, inv));
+        } else {
+            LQCD_DISPATCH_DS(L, k_clover_fields<D, S, 1><<<grid, kBlock, 0, L->ctx->stream>>>(
+                                    (const vec2_t<S>*)L->U[L->cur], L->g, d_out, inv));
+        }
         break;
     default:
         LQCD_DISPATCH_DS(L, k_magnetic_field<D, S><<<grid, kBlock, 0, L->ctx->stream>>>(
