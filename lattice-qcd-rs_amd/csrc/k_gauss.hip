@@ -618,8 +618,9 @@ static int gauss_blocks(int64_t n)
 // (or LQCD_GAUSS_TILES=0 pins the site walk): L0 divides 64 (whole x_0 rows
 // per wavefront), a tile L0 x a x b of kBlock sites with the rows of a
 // wavefront in one x_2 layer, a | L1, b | L2 and the smallest halo L0 (a + b);
-// C | L3 the largest of 32, 16, .., 1 that leaves min(1024, half the tiles of the
-// lattice) workgroups or more and at most kMaxReduceBlocks.
+// C | L3 the largest of 32, 16, .., 1 that leaves min(512, half the tiles of the
+// lattice) workgroups or more (512: one round of two workgroups per CU) and at
+// most kMaxReduceBlocks.
 static GaussTiles gauss_tiles(const Geom& g)
 {
     GaussTiles gt;
@@ -643,7 +644,7 @@ static GaussTiles gauss_tiles(const Geom& g)
     if (best_a == 0) return gt;
     const int64_t tiles = (int64_t)(L1 / best_a) * (L2 / best_b);
     const int64_t all = tiles * L3;
-    const int64_t target = all / 2 < 1024 ? all / 2 : 1024;
+    const int64_t target = all / 2 < 512 ? all / 2 : 512;
     int C = 0;
     for (int c : {32, 16, 8, 4, 2, 1})
         if (L3 % c == 0 && all / c >= target && all / c <= kMaxReduceBlocks) {

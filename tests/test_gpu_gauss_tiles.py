@@ -68,7 +68,7 @@ def test_gauss_tiles_vs_oracle(ext, cold):
 
 @pytest.mark.parametrize("precision", [lqcd.F32, lqcd.F64])
 def test_gauss_tiles_vs_site_walk_32x4(precision):
-    # 32^4: a = 2, b = 4, 16 x 8 tiles per slice, C = 4 (1024 workgroups)
+    # 32^4: a = 2, b = 4, 16 x 8 tiles per slice, C = 8 (512 workgroups)
     announce(f"gauss tiles vs site walk 32^4 precision={precision}")
     st = lqcd.LatticeStateDefault.new_determinist(1.0, 6.0, 32, 5, 4, precision)
     ef = lqcd.LatticeStateEFSyncDefault(st)
