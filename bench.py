@@ -77,10 +77,11 @@ REALS_KICK = REALS_FORCE
 REALS_KICK_R12 = 28
 # VALU instructions per site of the fused f32 exp-drift kernels by (MD variant,
 # two-row links), counted in their gfx950 assembly (hipcc -O3 --cuda-device-only -S
-# of k_md.hip: k_md_site_xcd<4, float, float, 1, R12> for variants 2 and 3,
-# k_md_site_tile<4, float, float, 1> for variant 4), and the issue rate of
+# of k_md.hip: k_md_site_xcd<4, float, float, 1, R12, 0> for variants 2 and 3,
+# k_md_site_tile<4, float, float, 1, 0> for variant 4; every branch counted, so an
+# upper bound of the executed count), and the issue rate of
 # the vector ALU: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per 4 cycles per SIMD
-VALU_PER_SITE = {(2, True): 7031, (3, True): 7031, (4, True): 7667, (2, False): 6902, (3, False): 6902}
+VALU_PER_SITE = {(2, True): 6971, (3, True): 6971, (4, True): 7703, (2, False): 6842, (3, False): 6842}
 VALU_WAVE_INSTR_PER_S = 256 * 4 * 2.4e9 / 4
 REALS_PLAQ = 18
 REALS_EENERGY = 8

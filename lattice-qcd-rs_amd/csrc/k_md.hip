@@ -207,7 +207,7 @@ void k_md_site_tile(const vec2_t<S>* __restrict__ U_in,
             const mat3<T> u = tile_r12<T>(tile, ts, mu, own);
             mat3<T> m;
             mul<false, false>(u, sig[mu], m);
-            im_trace_generators(m, f[mu]);
+            im_trace_combinations(m, f[mu]);
             if constexpr (SUMS != SUMS_NONE) acc.tr += re_trace(m);
         }
     }
@@ -216,8 +216,7 @@ void k_md_site_tile(const vec2_t<S>* __restrict__ U_in,
     for (int mu = 0; mu < D; ++mu) {
         load_e<T, S, kCacheStream>(E, g.npad, mu, s, e[mu]);
         if constexpr (SUMS == SUMS_BEFORE_KICK) acc.e2 += e_squares8<T>(e[mu]);
-#pragma unroll
-        for (int a = 0; a < 8; ++a) e[mu][a] = fma(kick, fscale * f[mu][a], e[mu][a]);
+        kick_generators(f[mu], kick * fscale, e[mu]);
         if constexpr (SUMS == SUMS_AFTER_KICK) acc.e2 += e_squares8<T>(e[mu]);
         store_e<T, S, kCacheStream>(E, g.npad, mu, s, e[mu]);
     }

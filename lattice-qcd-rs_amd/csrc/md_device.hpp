@@ -357,10 +357,9 @@ __device__ __forceinline__ void finish_link(const mat3<T>& u, const mat3<T>& sig
     mat3<T> m;
     mul<false, false>(u, sigma, m);
     T f[8], e[8];
-    im_trace_generators(m, f);
+    im_trace_combinations(m, f);
     load_e<T, S, kCacheStream>(E, g.npad, mu, s, e);
-#pragma unroll
-    for (int a = 0; a < 8; ++a) e[a] = fma(kick, fscale * f[a], e[a]);
+    kick_generators(f, kick * fscale, e);
     store_e<T, S, kCacheStream>(E, g.npad, mu, s, e);
     if constexpr (DR >= 0) drift_store<DR, R01, S, T>(u, e, dfac, U_out, g, mu, s);
 }
@@ -425,7 +424,8 @@ __device__ __forceinline__ void md_site(const vec2_t<S>* __restrict__ U_in, vec2
 #pragma unroll
         for (int mu = 0; mu < D; ++mu) own_put<T>(own, mu, load_link_r12<T, S>(U_in, g.npad, mu, s));
     }
-    // phase 1: the D forces (the staple sums die here)
+    // phase 1: the D forces (the staple sums die here), as the unscaled
+    // generator combinations: the kick folds in their factors
     T f[D][8];
     {
         mat3<T> sig[D];
@@ -436,7 +436,7 @@ __device__ __forceinline__ void md_site(const vec2_t<S>* __restrict__ U_in, vec2
 
             mat3<T> m;
             mul<false, false>(u, sig[mu], m);
-            im_trace_generators(m, f[mu]);
+            im_trace_combinations(m, f[mu]);
             if constexpr (SUMS != SUMS_NONE) acc->tr += re_trace(m);
         }
     }
@@ -446,8 +446,7 @@ __device__ __forceinline__ void md_site(const vec2_t<S>* __restrict__ U_in, vec2
     for (int mu = 0; mu < D; ++mu) {
         load_e<T, S, kCacheStream>(E, g.npad, mu, s, e[mu]);
         if constexpr (SUMS == SUMS_BEFORE_KICK) acc->e2 += e_squares8<T>(e[mu]);
-#pragma unroll
-        for (int a = 0; a < 8; ++a) e[mu][a] = fma(kick, fscale * f[mu][a], e[mu][a]);
+        kick_generators(f[mu], kick * fscale, e[mu]);
         if constexpr (SUMS == SUMS_AFTER_KICK) acc->e2 += e_squares8<T>(e[mu]);
         store_e<T, S, kCacheStream>(E, g.npad, mu, s, e[mu]);
     }

@@ -240,6 +240,33 @@ __device__ __forceinline__ void im_trace_generators(const mat3<T>& M, T* out)
     out[7] = r3 * (M.e[0].im + M.e[4].im - T(2) * M.e[8].im);
 }
 
+// The combinations whose multiples are the eight Im Tr(T^a M): out_a / 2 for
+// a < 7 and out_7 / (2 sqrt 3) are im_trace_generators(M); callers that scale
+// the projections anyway (the momentum kick) fold those factors into their
+// scale (kick_generators).
+template <typename T>
+__device__ __forceinline__ void im_trace_combinations(const mat3<T>& M, T* out)
+{
+    out[0] = M.e[1].im + M.e[3].im;
+    out[1] = M.e[1].re - M.e[3].re;
+    out[2] = M.e[0].im - M.e[4].im;
+    out[3] = M.e[2].im + M.e[6].im;
+    out[4] = M.e[2].re - M.e[6].re;
+    out[5] = M.e[5].im + M.e[7].im;
+    out[6] = M.e[5].re - M.e[7].re;
+    out[7] = M.e[0].im + M.e[4].im - T(2) * M.e[8].im;
+}
+
+// e_a += k Im Tr(T^a M) from the combinations c of im_trace_combinations
+template <typename T>
+__device__ __forceinline__ void kick_generators(const T* c, T k, T* e)
+{
+    const T kh = k * T(0.5), kr = k * T(0.28867513459481288225);
+#pragma unroll
+    for (int a = 0; a < 7; ++a) e[a] = fma(kh, c[a], e[a]);
+    e[7] = fma(kr, c[7], e[7]);
+}
+
 // 1/k! for k = 0..25
 __device__ __constant__ const double kInvFactorial[26] = {
     1.0,
@@ -362,8 +389,14 @@ __device__ __forceinline__ void exp_i(const T* v_in, mat3<T>& out)
     }
     T v[8];
 #pragma unroll
-    for (int a = 0; a < 8; ++a) v[a] = scale * v_in[a];
-    ss *= scale * scale;
+    for (int a = 0; a < 8; ++a) v[a] = v_in[a];
+    // scaling is exact (a power of two) and rare at molecular-dynamics step
+    // sizes: lanes without it skip the multiplications
+    if (m > 0) {
+#pragma unroll
+        for (int a = 0; a < 8; ++a) v[a] *= scale;
+        ss *= scale * scale;
+    }
     mat3<T> X, X2;
     adjoint_to_matrix(v, X);
     hermitian_square(X, X2);
