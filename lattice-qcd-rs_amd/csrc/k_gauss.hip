@@ -10,8 +10,10 @@
 // The iteration works on double-precision copies of E (for f32 storage as
 // well: the reference's stop rule, div <= N * 320 * eps, is far below what
 // an f32 field can represent).  EField::gauss keeps G as double planes
-// (k_gauss); the projection's steps compute it where they need it
-// (k_gauss_fused).
+// (k_gauss_tiles, or k_gauss for lattices without the streamed tiling); the
+// projection's steps compute it where they need it (k_gauss_stream, or
+// k_gauss_fused), each pair walking the sites in the same order so that the
+// deviation is summed the same way in both.
 #include <cstdlib>
 
 #include "dispatch.hpp"
