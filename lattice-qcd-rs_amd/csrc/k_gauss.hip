@@ -311,6 +311,10 @@ __device__ __forceinline__ herm3 herm_get(const double* base, int n, int i)
     return h;
 }
 
+// The projection steps store the new E non-temporally: the next step, a
+// launch of its own, reads it, and the L2 is better left to the E and U
+// that neighbouring sites read again within the step.
+
 // One project_to_gauss_step of the whole field with G computed on the fly,
 // for any lattice: Ed_out = step(Ed_in), and the block partials of
 // div(Ed_in).  The kernel walks the lattice in tiles of kBlock consecutive
@@ -362,7 +366,7 @@ __global__ __launch_bounds__(kBlock) void k_gauss_fused(const vec2_t<S>* __restr
                 double e[8];
                 load_e<double, double>(Ed_in, g.npad, i, s, e);
                 gauss_step_link(u, G, Gn, e);
-                store_e<double, double>(Ed_out, g.npad, i, s, e);
+                store_e<double, double, kCacheStream>(Ed_out, g.npad, i, s, e);
             }
         }
         // the next tile overwrites gl
@@ -481,7 +485,7 @@ __device__ __forceinline__ void gauss_step_slice(const vec2_t<S>* __restrict__ U
 #pragma unroll
         for (int a8 = 0; a8 < 8; ++a8) e[a8] = ek[i][a8];
         gauss_step_link(u, G, Gn, e);
-        store_e<double, double>(Ed_out, g.npad, i, s, e);
+        store_e<double, double, kCacheStream>(Ed_out, g.npad, i, s, e);
     }
 }
 
@@ -563,7 +567,7 @@ __global__ __launch_bounds__(kBlock) void k_gauss_stream(const vec2_t<S>* __rest
             double e[8];
             load_e<double, double>(Ed_in, g.npad, 3, s, e);
             gauss_step_link(u, G, Gn, e);
-            store_e<double, double>(Ed_out, g.npad, 3, s, e);
+            store_e<double, double, kCacheStream>(Ed_out, g.npad, 3, s, e);
         }
         if (more)
             gauss_step_slice<S>(U, Ed_out, g, rn, halo, nh, tile_site_index(g, on), tid, j0, j1, j2, in1, in2, ek);
