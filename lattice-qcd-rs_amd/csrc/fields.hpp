@@ -82,7 +82,7 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off
     }
 }
 
-template <typename T, typename S>
+template <typename T, typename S, int CP = kCacheDefault>
 __device__ __forceinline__ mat3<T> load_link(const vec2_t<S>* __restrict__ U, int64_t npad, int mu,
                                              int64_t s)
 {
@@ -93,7 +93,7 @@ __device__ __forceinline__ mat3<T> load_link(const vec2_t<S>* __restrict__ U, in
     mat3<T> m;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-        const E v = buf_load<E>(r, off, (uint32_t)k * plane);
+        const E v = buf_load<E, CP>(r, off, (uint32_t)k * plane);
         m.e[k] = {T(v.x), T(v.y)};
     }
     return m;

@@ -480,7 +480,8 @@ __device__ __forceinline__ void gauss_step_slice(const vec2_t<S>* __restrict__ U
         if (i == 0) Gn = herm_get(rc, kBlock, j0);
         else if (i == 1) Gn = in1 ? herm_get(rc, kBlock, j1) : herm_get(halo, nh, j1);
         else Gn = in2 ? herm_get(rc, kBlock, j2) : herm_get(halo, nh, j2);
-        const mat3<double> u = load_link<double, S>(U, g.npad, i, s);
+        // the last use of this link in the launch: loaded evict-first
+        const mat3<double> u = load_link<double, S, kCacheStream>(U, g.npad, i, s);
         double e[8];
 #pragma unroll
         for (int a8 = 0; a8 < 8; ++a8) e[a8] = ek[i][a8];
