@@ -313,8 +313,10 @@ int lqcd_field_strength_device(lqcd_lattice* lat, int what, double* d_out, size_
 enum { LQCD_FORMAT_BINCODE = 0, LQCD_FORMAT_JSON = 1 };
 int lqcd_state_write(lqcd_lattice* lat, double beta, const char* path, int format);
 int lqcd_state_ef_write(lqcd_lattice* lat, double beta, uint64_t t, const char* path, int format);
-/* parses a whole file (with_efield: the EFSync layout) and returns its lattice
- * spacing, points per direction, beta and t (t may be NULL); needs no device */
+/* checks a whole file (with_efield: the EFSync layout; bincode: every length
+ * prefix against the file size, which is the whole check for fixed-width
+ * fields; JSON: a full parse) and returns its lattice spacing, points per
+ * direction, beta and t (t may be NULL); needs no device */
 int lqcd_state_read_header(const char* path, int format, int with_efield, int dim, double* size,
                            size_t* points, double* beta, uint64_t* t);
 /* the same parse into host buffers: links n_links x 18 (link index site*D+mu,
@@ -322,7 +324,9 @@ int lqcd_state_read_header(const char* path, int format, int with_efield, int di
 int lqcd_state_read_host(const char* path, int format, int with_efield, int dim, double* links,
                          size_t n_links, double* efield);
 /* loads the links (and E) of a file into a lattice of the same shape and
- * spacing; beta and t (may be NULL) receive the stored values */
+ * spacing; beta and t (may be NULL) receive the stored values.  The whole
+ * file is read before the lattice is overwritten: a file that is refused
+ * leaves the lattice as it was */
 int lqcd_state_read(lqcd_lattice* lat, const char* path, int format, int with_efield, double* beta,
                     uint64_t* t);
 

@@ -76,3 +76,31 @@ def test_non_hypercubic_state_is_refused(tmp_path):
     st = lqcd.LatticeStateDefault(lqcd.LatticeCyclic(1.0, [4, 4, 6], 3), 6.0, ol.cold_start())
     with pytest.raises(lqcd.LqcdError):
         st.write(tmp_path / "x")
+
+
+@pytest.mark.parametrize("precision", [lqcd.F64, lqcd.F32])
+def test_efsync_bincode_through_several_chunks(tmp_path, precision):
+    # 16^4: 37.7 MB of links and 16.8 MB of E, each block several 8 MB pinned
+    # chunks between the file and HBM, the last one partial
+    announce(f"EFSync bincode 16^4 write/read precision={precision}")
+    st = lqcd.LatticeStateDefault.new_determinist(1.0, 6.0, 16, 21, 4, precision)
+    ef = lqcd.LatticeStateEFSyncDefault.new_random_e_state(st, 9, 0, project=False)
+    u, e = ef.link_matrix(), ef.e_field()
+    p = tmp_path / "ef16"
+    ef.write(p)
+    size, points, beta, links, e2, t = parse_ef_bincode(p.read_bytes(), 4)
+    assert (size, points, beta, t) == (1.0, 16, 6.0, ef.t())
+    assert np.array_equal(links, u) and np.array_equal(e2, e)
+    back = lqcd.LatticeStateEFSyncDefault.read(p, 4, F.FORMAT_BINCODE, precision)
+    assert np.array_equal(back.link_matrix(), u) and np.array_equal(back.e_field(), e)
+    # refused files leave the lattice as it was: a truncated file (I/O error)
+    # and a file of another lattice (incompatible size)
+    short = tmp_path / "short"
+    short.write_bytes(p.read_bytes()[:-8])
+    other = tmp_path / "other"
+    lqcd.LatticeStateEFSyncDefault.new_random_e_state(
+        lqcd.LatticeStateDefault.new_determinist(1.0, 6.0, 8, 3, 4, precision), 4, 0, project=False).write(other)
+    h = back.lattice_state().handle
+    for path, code in ((short, F.ERR_IO), (other, F.ERR_INCOMPATIBLE_SIZE)):
+        assert F.lib().lqcd_state_read(h, str(path).encode(), F.FORMAT_BINCODE, 1, None, None) == code
+        assert np.array_equal(back.link_matrix(), u) and np.array_equal(back.e_field(), e)
