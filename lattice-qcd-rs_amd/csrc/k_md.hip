@@ -33,23 +33,47 @@
 
 namespace lqcd {
 
+// derivative_e (src/simulation/state.rs:1420-1448) of every link, times
+// oscale, in the host layout [site][mu][8]: one thread per site, the D staple
+// sums sharing their products (staple_sums_site).  A block's output is one
+// contiguous run of kForceBlock * D * 8 doubles; the threads put their
+// values into an LDS image of it (rows of D * 8 + 1 doubles, so that the
+// lanes' 8-byte writes fall into different banks) and the block stores the
+// image as whole 16-byte words, lane after lane.
+constexpr int kForceBlock = 128;
+
 template <int D, typename S, typename T>
-__global__ __launch_bounds__(kBlock) void k_force_store(const vec2_t<S>* __restrict__ U, Geom g,
-                                                        T fscale, double* __restrict__ out, double oscale)
+__global__ __launch_bounds__(kForceBlock) void k_force_store(const vec2_t<S>* __restrict__ U, Geom g,
+                                                             T fscale, double* __restrict__ out,
+                                                             double oscale)
 {
-    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const int mu = blockIdx.y;
-    if (s >= g.nsite) return;
-    const Neighbours<D> nb = neighbours<D>(g, s);
-    mat3<T> sigma, m;
-    staple_sum<D, S, T>(U, g, mu, s, nb, sigma);
-    const mat3<T> u = load_link<T, S>(U, g.npad, mu, s);
-    mul<false, false>(u, sigma, m);
-    T f[8];
-    im_trace_generators(m, f);
-    double* p = out + (s * D + mu) * 8;
+    constexpr int kRow = D * 8 + 1;
+    __shared__ double img[kForceBlock * kRow];
+    const int64_t s0 = (int64_t)blockIdx.x * kForceBlock;
+    const int64_t s = s0 + threadIdx.x;
+    if (s < g.nsite) {
+        const Neighbours<D> nb = neighbours<D>(g, s);
+        mat3<T> sig[D];
+        staple_sums_site<D, S, T>(U, g, s, nb, sig);
 #pragma unroll
-    for (int a = 0; a < 8; ++a) p[a] = oscale * (double)(fscale * f[a]);
+        for (int mu = 0; mu < D; ++mu) {
+            const mat3<T> u = load_link<T, S>(U, g.npad, mu, s);
+            mat3<T> m;
+            mul<false, false>(u, sig[mu], m);
+            T f[8];
+            im_trace_generators(m, f);
+#pragma unroll
+            for (int a = 0; a < 8; ++a) img[threadIdx.x * kRow + mu * 8 + a] = oscale * (double)(fscale * f[a]);
+        }
+    }
+    __syncthreads();
+    // word w of the block's run: site w / (D * 4), doubles 2 (w % (D * 4)) + 0, 1
+    const int n = (int)min<int64_t>(kForceBlock, g.nsite - s0);
+    double2* o = reinterpret_cast<double2*>(out + s0 * D * 8);
+    for (int w = threadIdx.x; w < n * D * 4; w += kForceBlock) {
+        const int site = w / (D * 4), k = 2 * (w % (D * 4));
+        o[w] = make_double2(img[site * kRow + k], img[site * kRow + k + 1]);
+    }
 }
 
 // derivative_u (src/simulation/state.rs:1407-1417) of every link, double
@@ -370,8 +394,9 @@ hipError_t launch_force_store(const lqcd_lattice* L, const void* U, double* d_ou
 {
     lqcd_prof_event ev;
     prof_begin(L->ctx, PROF_FORCE_STORE, &ev);
-    LQCD_DISPATCH_DS(L, k_force_store<D, S, S><<<link_grid(L), kBlock, 0, L->ctx->stream>>>(
-                            (const vec2_t<S>*)U, L->g, (S)md_force_scale(L->size), d_out, scale));
+    LQCD_DISPATCH_DS(L, k_force_store<D, S, S><<<grid_for(L->g.nsite, kForceBlock), kForceBlock, 0,
+                                                L->ctx->stream>>>((const vec2_t<S>*)U, L->g,
+                                                                  (S)md_force_scale(L->size), d_out, scale));
     prof_end(L->ctx, &ev);
     return hipGetLastError();
 }
