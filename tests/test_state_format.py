@@ -131,3 +131,34 @@ def test_oversized_length_prefixes_are_refused(tmp_path):
         good_ef = ef_bincode(1.0, points, 6.0, links, e, 3, dim)
         p.write_bytes(struct.pack("<Q", n) + good_ef[8:])
         assert header(p, F.FORMAT_BINCODE, 1, dim)[0] == F.ERR_IO, n
+
+
+def test_every_truncation_of_a_bincode_state_is_refused(tmp_path):
+    # the length prefixes are checked against the file size before any block
+    # is read: every proper prefix of a well-formed file is an I/O error, for
+    # the header read and for the read into host buffers
+    dim, points = 3, 2
+    links, e, nlink = random_state(dim, points, 6)
+    good = ef_bincode(0.75, points, 5.0, links, e, 9, dim)
+    p = tmp_path / "cut"
+    e_end = 8 + nlink * 64
+    cuts = set(range(0, len(good), 37)) | {0, 7, 8, 9, e_end - 1, e_end, e_end + 7, e_end + 8, e_end + 39,
+                                           e_end + 40, len(good) - 144, len(good) - 1}
+    for n in sorted(cuts):
+        p.write_bytes(good[:n])
+        assert header(p, F.FORMAT_BINCODE, 1, dim)[0] == F.ERR_IO, n
+        assert read_host(p, F.FORMAT_BINCODE, 1, dim, nlink)[0] == F.ERR_IO, n
+    p.write_bytes(good)
+    rc, size, pts, beta, t = header(p, F.FORMAT_BINCODE, 1, dim)
+    assert rc == F.OK and (size, pts, beta, t) == (0.75, points, 5.0, 9)
+
+
+def test_bincode_e_field_of_another_lattice_is_refused(tmp_path):
+    # an E block whose site count fits the file but not the lattice: the
+    # lengths are consistent, the shape is not
+    dim, points = 3, 2
+    links, e, nlink = random_state(dim, points, 7)
+    p = tmp_path / "e"
+    p.write_bytes(ef_bincode(1.0, points, 6.0, links, e[: nlink - dim], 1, dim))
+    assert header(p, F.FORMAT_BINCODE, 1, dim)[0] == F.ERR_INCOMPATIBLE_SIZE
+    assert read_host(p, F.FORMAT_BINCODE, 1, dim, nlink)[0] == F.ERR_INCOMPATIBLE_SIZE
