@@ -524,6 +524,11 @@ int lqcd_stats_covariance(lqcd_ctx* ctx, const double* d_x, size_t nx, const dou
  * achievable HBM bandwidth of the device, and a reference of known size for
  * byte counters.  No counterpart in the reference. */
 int lqcd_stream_copy(lqcd_ctx* ctx, size_t bytes, int iters, double* gb_per_s);
+/* Reads `bytes` (a multiple of 16) of one device buffer `iters` times with a
+ * streaming kernel and returns the read rate in GB/s: the reference point for
+ * kernels that only read (a read-only stream can pass the copy rate without
+ * any cache serving it).  No counterpart in the reference. */
+int lqcd_stream_read(lqcd_ctx* ctx, size_t bytes, int iters, double* gb_per_s);
 
 /* Record HIP events around every launch of the kinds in `mask` (bit = kind)
  * on the context's stream; mask 0 stops recording.  Clears past records. */

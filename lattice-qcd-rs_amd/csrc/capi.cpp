@@ -1238,6 +1238,33 @@ int lqcd_stream_copy(lqcd_ctx* ctx, size_t bytes, int iters, double* gb_per_s)
     return LQCD_OK;
 }
 
+int lqcd_stream_read(lqcd_ctx* ctx, size_t bytes, int iters, double* gb_per_s)
+{
+    CHECK_ARG(ctx && gb_per_s, "null argument");
+    CHECK_ARG(bytes >= 16 && bytes % 16 == 0, "bytes must be a positive multiple of 16");
+    CHECK_ARG(iters > 0, "iters must be positive");
+    DeviceGuard dg(ctx->device);
+    DevBuf a, sink;
+    HIP_TRY(hipMalloc(&a.p, bytes));
+    HIP_TRY(hipMalloc(&sink.p, sizeof(uint32_t) * (size_t)kStreamReadBlocks * kBlock));
+    HIP_TRY(hipMemsetAsync(a.p, 0, bytes, ctx->stream));  // first touch
+    HIP_TRY(launch_stream_read(ctx->stream, a.p, bytes, (uint32_t*)sink.p));
+    hipEvent_t t0, t1;
+    HIP_TRY(hipEventCreate(&t0));
+    HIP_TRY(hipEventCreate(&t1));
+    hipError_t e = hipEventRecord(t0, ctx->stream);
+    for (int i = 0; i < iters && e == hipSuccess; ++i) e = launch_stream_read(ctx->stream, a.p, bytes, (uint32_t*)sink.p);
+    if (e == hipSuccess) e = hipEventRecord(t1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(t1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (e != hipSuccess) return fail(LQCD_ERR_DEVICE, std::string("stream read: ") + hipGetErrorString(e));
+    *gb_per_s = (double)bytes * iters / (ms * 1e-3) / 1e9;
+    return LQCD_OK;
+}
+
 int lqcd_md_autotune(lqcd_lattice* L, int drift, int* variant)
 {
     CHECK_ARG(L, "lattice is null");

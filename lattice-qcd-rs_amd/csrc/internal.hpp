@@ -114,6 +114,10 @@ hipError_t launch_reunitarize(const lqcd_lattice* L, void* U);
 hipError_t launch_complete_row2(const lqcd_lattice* L, void* U);
 hipError_t launch_links_random(const lqcd_lattice* L, void* U, uint64_t seed);
 hipError_t launch_stream_copy(hipStream_t stream, const void* src, void* dst, size_t bytes);
+// read-only streaming probe: grid of kStreamReadBlocks x kBlock threads, sink
+// holds one uint32 per thread
+constexpr int kStreamReadBlocks = 8192;
+hipError_t launch_stream_read(hipStream_t stream, const void* src, size_t bytes, uint32_t* sink);
 
 // ---- observables (k_observables.hip) ----
 // out[0..1] = sum Tr P (complex), out[2] = sum (1 - Re Tr P / 3)

@@ -154,6 +154,36 @@ hipError_t launch_stream_copy(hipStream_t stream, const void* src, void* dst, si
     return hipGetLastError();
 }
 
+// Streaming read of n 16-byte words, four loads in flight per thread: the
+// reference point for kernels that only read.  Each thread folds its words
+// with xor and stores the fold (into its own slot of sink, grid x kBlock
+// words) only when it equals `never`, so the loads stay and nothing is
+// written in practice.
+__global__ __launch_bounds__(kBlock) void k_stream_read(const uint4* __restrict__ a, int64_t n, uint32_t never,
+                                                         uint32_t* __restrict__ sink)
+{
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    uint32_t acc = 0;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const uint4 v0 = a[i], v1 = a[i + stride], v2 = a[i + 2 * stride], v3 = a[i + 3 * stride];
+        acc ^= (v0.x ^ v0.y ^ v0.z ^ v0.w) ^ (v1.x ^ v1.y ^ v1.z ^ v1.w) ^ (v2.x ^ v2.y ^ v2.z ^ v2.w) ^
+               (v3.x ^ v3.y ^ v3.z ^ v3.w);
+    }
+    for (; i < n; i += stride) {
+        const uint4 v = a[i];
+        acc ^= v.x ^ v.y ^ v.z ^ v.w;
+    }
+    if (acc == never) sink[(int64_t)blockIdx.x * kBlock + threadIdx.x] = acc;
+}
+
+hipError_t launch_stream_read(hipStream_t stream, const void* src, size_t bytes, uint32_t* sink)
+{
+    const int64_t n = (int64_t)(bytes / 16);
+    k_stream_read<<<kStreamReadBlocks, kBlock, 0, stream>>>((const uint4*)src, n, 0x9e3779b9u, sink);
+    return hipGetLastError();
+}
+
 static dim3 link_grid(const lqcd_lattice* L)
 {
     return dim3(grid_for(L->g.nsite), (unsigned)L->g.dim);

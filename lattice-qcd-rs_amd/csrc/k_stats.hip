@@ -4,7 +4,9 @@
 // resident in device memory.  Each launch is a fixed grid of partial sums
 // (four independent accumulators per thread, grid-stride), wave64 butterflies,
 // then one block summing the partials in block order: results are the same
-// run to run for a given n.
+// run to run for a given n and buffer alignment.  Buffers aligned to 16 bytes
+// are read 16 bytes per lane (two reals or one complex value), which keeps
+// twice the bytes in flight per load; others a double at a time.
 #include <hip/hip_runtime.h>
 
 #include "dispatch.hpp"
@@ -57,11 +59,11 @@ struct StatsK<STATS_COV> {
     static constexpr int N = 3;
 };
 
+// the sums of one element a (and b = y's element for STATS_COV) added to acc
 template <bool CPLX, int OP>
-__device__ __forceinline__ void stats_terms(const double* __restrict__ x, const double* __restrict__ y,
-                                            int64_t i, cval m, cval (&t)[StatsK<OP>::N])
+__device__ __forceinline__ void add_terms(cval a, cval b, cval m, double* acc)
 {
-    const cval a = value_at<CPLX>(x, i);
+    cval t[StatsK<OP>::N];
     if constexpr (OP == STATS_MOMENTS) {
         t[0] = a;
         t[1] = cmul(a, a);
@@ -69,14 +71,41 @@ __device__ __forceinline__ void stats_terms(const double* __restrict__ x, const 
         const cval d = {a.re - m.re, a.im - m.im};
         t[0] = cmul(d, d);
     } else {
-        const cval b = value_at<CPLX>(y, i);
         t[0] = a;
         t[1] = b;
         t[2] = cmul(a, b);
     }
+#pragma unroll
+    for (int k = 0; k < StatsK<OP>::N; ++k) {
+        acc[CPLX ? 2 * k : k] += t[k].re;
+        if constexpr (CPLX) acc[2 * k + 1] += t[k].im;
+    }
 }
 
-template <bool CPLX, int OP>
+// unit u of the grid-stride walk: with VEC one 16-byte word (two reals, or
+// one complex value), else one element
+template <bool CPLX, int OP, bool VEC>
+__device__ __forceinline__ void add_unit(const double* __restrict__ x, const double* __restrict__ y, int64_t u,
+                                         cval m, double* acc)
+{
+    if constexpr (VEC) {
+        const double2 p = reinterpret_cast<const double2*>(x)[u];
+        double2 q = make_double2(0.0, 0.0);
+        if constexpr (OP == STATS_COV) q = reinterpret_cast<const double2*>(y)[u];
+        if constexpr (CPLX) {
+            add_terms<CPLX, OP>({p.x, p.y}, {q.x, q.y}, m, acc);
+        } else {
+            add_terms<CPLX, OP>({p.x, 0.0}, {q.x, 0.0}, m, acc);
+            add_terms<CPLX, OP>({p.y, 0.0}, {q.y, 0.0}, m, acc);
+        }
+    } else {
+        cval b = {0.0, 0.0};
+        if constexpr (OP == STATS_COV) b = value_at<CPLX>(y, u);
+        add_terms<CPLX, OP>(value_at<CPLX>(x, u), b, m, acc);
+    }
+}
+
+template <bool CPLX, int OP, bool VEC>
 __global__ __launch_bounds__(kBlock) void k_stats_partials(const double* __restrict__ x,
                                                            const double* __restrict__ y, int64_t n,
                                                            const double* __restrict__ center,
@@ -95,30 +124,19 @@ __global__ __launch_bounds__(kBlock) void k_stats_partials(const double* __restr
     for (int u = 0; u < kStatsUnroll; ++u)
 #pragma unroll
         for (int k = 0; k < K; ++k) acc[u][k] = 0.0;
+    // units of the walk: pairs of reals when VEC reads real values two at a
+    // time (an odd last element is thread 0's), else elements
+    const int64_t units = (VEC && !CPLX) ? n / 2 : n;
     const int64_t step = (int64_t)gridDim.x * kBlock;
     int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     // kStatsUnroll independent loads in flight per thread
-    for (; i + (kStatsUnroll - 1) * step < n; i += kStatsUnroll * step) {
+    for (; i + (kStatsUnroll - 1) * step < units; i += kStatsUnroll * step) {
 #pragma unroll
-        for (int u = 0; u < kStatsUnroll; ++u) {
-            cval t[NK];
-            stats_terms<CPLX, OP>(x, y, i + u * step, m, t);
-#pragma unroll
-            for (int k = 0; k < NK; ++k) {
-                acc[u][CPLX ? 2 * k : k] += t[k].re;
-                if constexpr (CPLX) acc[u][2 * k + 1] += t[k].im;
-            }
-        }
+        for (int u = 0; u < kStatsUnroll; ++u) add_unit<CPLX, OP, VEC>(x, y, i + u * step, m, acc[u]);
     }
-    for (; i < n; i += step) {
-        cval t[NK];
-        stats_terms<CPLX, OP>(x, y, i, m, t);
-#pragma unroll
-        for (int k = 0; k < NK; ++k) {
-            acc[0][CPLX ? 2 * k : k] += t[k].re;
-            if constexpr (CPLX) acc[0][2 * k + 1] += t[k].im;
-        }
-    }
+    for (; i < units; i += step) add_unit<CPLX, OP, VEC>(x, y, i, m, acc[0]);
+    if constexpr (VEC && !CPLX)
+        if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) add_unit<CPLX, OP, false>(x, y, n - 1, m, acc[0]);
     double v[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) v[k] = (acc[0][k] + acc[1][k]) + (acc[2][k] + acc[3][k]);
@@ -145,7 +163,11 @@ hipError_t launch_stats_t(lqcd_ctx* ctx, const double* x, const double* y, int64
     constexpr int K = CPLX ? 2 * StatsK<OP>::N : StatsK<OP>::N;
     lqcd_prof_event ev;
     prof_begin(ctx, PROF_STATS, &ev);
-    k_stats_partials<CPLX, OP><<<nblocks, kBlock, 0, ctx->stream>>>(x, y, n, center, partials);
+    const bool vec = ((uintptr_t)x & 15) == 0 && (OP != STATS_COV || ((uintptr_t)y & 15) == 0);
+    if (vec)
+        k_stats_partials<CPLX, OP, true><<<nblocks, kBlock, 0, ctx->stream>>>(x, y, n, center, partials);
+    else
+        k_stats_partials<CPLX, OP, false><<<nblocks, kBlock, 0, ctx->stream>>>(x, y, n, center, partials);
     prof_end(ctx, &ev);
     k_stats_final<K><<<1, kBlock, 0, ctx->stream>>>(partials, nblocks, out);
     return hipGetLastError();

@@ -168,6 +168,7 @@ SIGNATURES = {
     "lqcd_overrelaxation_reverse_sweep": (_I, [_P]),
     "lqcd_normalize_link_matrices": (_I, [_P]),
     "lqcd_stream_copy": (_I, [_P, _S, _I, C.POINTER(_D)]),
+    "lqcd_stream_read": (_I, [_P, _S, _I, C.POINTER(_D)]),
     "lqcd_md_autotune": (_I, [_P, _I, C.POINTER(C.c_int)]),
     "lqcd_md_variant_set": (_I, [_P, _I]),
     "lqcd_md_variant_get": (_I, [_P, C.POINTER(C.c_int)]),

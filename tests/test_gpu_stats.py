@@ -141,6 +141,32 @@ def test_edge_cases_and_buffers():
         S.DeviceValues.from_buffer(buf, 11, True, offset=8)
 
 
+@pytest.mark.parametrize("n", [2, 3, 1_000_001])
+def test_aligned_and_unaligned_reads_agree(n):
+    # buffers aligned to 16 bytes are read two reals per lane, 8-byte aligned
+    # ones a double at a time: the same sums up to the order of the additions,
+    # and covariance of one of each takes the double-at-a-time path
+    announce(f"statistics aligned / unaligned reads n={n}")
+    rng = np.random.default_rng(300 + n)
+    a = rng.normal(0.7, 1.3, n + 1)
+    b = rng.normal(-0.2, 0.9, n + 1)
+    bufs = []
+    for arr in (a, b):
+        buf = lqcd._ffi.DeviceBuffer(arr.nbytes)
+        buf.fill_from(arr)
+        bufs.append(buf)
+    xu = S.DeviceValues.from_buffer(bufs[0], n, False, offset=8)   # a[1:], 8-byte aligned
+    yu = S.DeviceValues.from_buffer(bufs[1], n, False, offset=8)
+    xa, ya = S.DeviceValues.from_numpy(a[1:].copy()), S.DeviceValues.from_numpy(b[1:].copy())
+    om, ov = O.stats_mean_and_variance(a[1:])
+    for v in (xu, xa):
+        m, var = S.mean_and_variance(v)
+        assert close(m, om, 1e-15, 1e-13) and close(var, ov, 1e-13)
+    c_ref = O.stats_covariance(a[1:], b[1:], False)
+    for x, y in ((xa, ya), (xa, yu), (xu, yu)):
+        assert close(S.covariance(x, y), c_ref, 1e-15, 1e-12)
+
+
 def test_statistics_of_a_device_field():
     # the statistics of F_mu_nu components left on the device by
     # lqcd_field_strength_device, against the same field on the host
