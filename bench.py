@@ -440,6 +440,16 @@ def bench_adjoint(args, lqcd):
     # backward: read U (18 reals), write the 8 gradient components in f64
     bwd = kernel_rate(ctx, F.KERNEL_FORCE, nlink * (REALS_PLAQ + 8) * 8)
     ctx.profile_enable(0)
+    # the same pairs with the gradient left in a device buffer
+    gbuf = F.DeviceBuffer(nlink * 8 * 8)
+    st.wilson_backward_device(gbuf)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n_fb):
+        st.wilson_forward()
+        st.wilson_backward_device(gbuf)
+    t_fb_dev = time.perf_counter() - t0
+    del gbuf
     integ = lqcd.SymplecticEulerGpu(lqcd.DRIFT_ADDITIVE, lqcd.SCHEME_REFERENCE)
     # kernel times from profiled trajectories on a copy of the state (with
     # profiling on, the reference scheme launches eagerly, not from a graph)
@@ -466,7 +476,8 @@ def bench_adjoint(args, lqcd):
     seconds = time.perf_counter() - t0
     value = nlink * md * n_traj / seconds
     cfg = {"workload": ("16^4 D=4 FP64 beta=5.7, hot start (device stream): 50 Wilson forward + backward "
-                        "pairs (action to the host, gradient [link][8] to the host), then 50 HMC trajectories "
+                        "pairs (action to the host, gradient [link][8] to the host; then again with the gradient "
+                        "left in a device buffer), then 50 HMC trajectories "
                         "of 20 kick-drift-kick steps dt=0.01 with the reference's additive drift, momenta "
                         "N(0, 0.5/beta) without the Gauss projection"),
            "lattice": [L] * 4, "links": nlink, "md_steps": md, "dt": dt, "beta": beta}
@@ -493,6 +504,8 @@ def bench_adjoint(args, lqcd):
         mean_abs_dH=float(np.mean(np.abs(dh))), plaquette=st.average_trace_plaquette().real / 3.0,
         forward_backward_per_s=round(n_fb / t_fb, 2),
         forward_backward_ms=round(1e3 * t_fb / n_fb, 3),
+        forward_backward_device_per_s=round(n_fb / t_fb_dev, 2),
+        forward_backward_device_ms=round(1e3 * t_fb_dev / n_fb, 3),
         kernels={"wilson_forward": fwd, "wilson_backward": bwd, "kick_drift": kd, "kick": kick})
 
 

@@ -206,6 +206,10 @@ int lqcd_wilson_forward(lqcd_lattice* lat, double beta, double* action);
  * equals -(beta a / sqrt 6) * derivative_e (src/simulation/state.rs:1420-1448).
  * Staples are recomputed, nothing is stored by the forward pass. */
 int lqcd_wilson_backward(lqcd_lattice* lat, double beta, double* grad, size_t n_links);
+/* The same gradient into a caller-owned device buffer (d_grad: device memory
+ * of the lattice's GPU, n_links x 8 doubles), for callers that keep it on the
+ * device: no host copy.  No counterpart in the reference. */
+int lqcd_wilson_backward_device(lqcd_lattice* lat, double beta, double* d_grad, size_t n_links);
 
 /* ----------------------------------------------------------- Metropolis */
 typedef struct {

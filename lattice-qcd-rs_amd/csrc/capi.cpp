@@ -1079,6 +1079,17 @@ int lqcd_wilson_backward(lqcd_lattice* L, double beta, double* grad, size_t n_li
     return force_to_host(L, grad, n_links, -(beta * L->size) / std::sqrt(6.0));
 }
 
+int lqcd_wilson_backward_device(lqcd_lattice* L, double beta, double* d_grad, size_t n_links)
+{
+    CHECK_ARG(std::isfinite(beta), "beta must be finite");
+    CHECK_ARG(L && d_grad, "null argument");
+    if (int rc = check_links(L, n_links)) return rc;
+    DeviceGuard dg(L->ctx->device);
+    HIP_TRY(launch_force_store(L, L->U[L->cur], d_grad, -(beta * L->size) / std::sqrt(6.0)));
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    return LQCD_OK;
+}
+
 static int metropolis_prepare(lqcd_lattice* L, double beta, size_t n_update)
 {
     CHECK_ARG(n_update > 0, "n_update must be positive");

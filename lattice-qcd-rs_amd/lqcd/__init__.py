@@ -567,6 +567,15 @@ class LatticeStateDefault:
         _raise(F.lib().lqcd_wilson_backward(self.handle, self._beta, F.ptr(out), self._dev.nlink))
         return out
 
+    def wilson_backward_device(self, buf: "F.DeviceBuffer") -> "F.DeviceBuffer":
+        """The backward pass into a device buffer of at least nlink * 8 doubles
+        (``lqcd_wilson_backward_device``): the gradient stays on the GPU."""
+        if buf.nbytes < self._dev.nlink * 8 * 8:
+            raise ValueError("device buffer smaller than nlink * 8 doubles")
+        _raise(F.lib().lqcd_wilson_backward_device(self.handle, self._beta, C.c_void_p(buf.ptr),
+                                                   self._dev.nlink))
+        return buf
+
 
 class LatticeStateEFSyncDefault:
     """``LatticeStateEFSyncDefault<LatticeStateDefault<D>, D>``

@@ -137,6 +137,7 @@ SIGNATURES = {
     "lqcd_hmc_trajectory": (_I, [_P, _D, _D, _S, _I, _I, _U64, C.POINTER(HmcStats)]),
     "lqcd_wilson_forward": (_I, [_P, _D, C.POINTER(_D)]),
     "lqcd_wilson_backward": (_I, [_P, _D, _P, _S]),
+    "lqcd_wilson_backward_device": (_I, [_P, _D, _P, _S]),
     "lqcd_metropolis_sweep": (_I, [_P, _D, _S, _P, _P, C.POINTER(MhStats)]),
     "lqcd_metropolis_sweep_random": (_I, [_P, _D, _S, _D, _U64, _U64, C.POINTER(MhStats)]),
     "lqcd_heat_bath_sweep": (_I, [_P, _D, _U64, _U64, C.POINTER(C.c_size_t)]),

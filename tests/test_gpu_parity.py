@@ -83,6 +83,25 @@ def test_derivative_e_and_backward(dim, ext):
     assert np.max(np.abs(grad - gref)) <= 1e-12 * np.max(np.abs(gref))
 
 
+@pytest.mark.parametrize("dim,ext", [(4, [4, 6, 2, 8]), (3, [6, 6, 6])])
+def test_wilson_backward_into_device_buffer(dim, ext):
+    # the same kernel and scale as the host entry point: bit for bit
+    announce(f"wilson backward into a device buffer D={dim} ext={ext}")
+    import ctypes as C
+    ol, u, st = make_state(dim, ext, 5.7, size=0.5)
+    grad = st.wilson_backward()
+    buf = F.DeviceBuffer(grad.nbytes)
+    st.wilson_backward_device(buf)
+    assert np.array_equal(buf.to_numpy().reshape(grad.shape), grad)
+    assert np.max(np.abs(grad - ol.wilson_backward(u, 5.7))) <= 1e-12 * np.max(np.abs(grad))
+    lib = F.lib()
+    assert lib.lqcd_wilson_backward_device(st.handle, 5.7, C.c_void_p(buf.ptr), grad.shape[0] - 1) \
+        == F.ERR_INCOMPATIBLE_SIZE
+    assert lib.lqcd_wilson_backward_device(st.handle, 5.7, None, grad.shape[0]) == F.ERR_INVALID_ARGUMENT
+    assert lib.lqcd_wilson_backward_device(st.handle, float("nan"), C.c_void_p(buf.ptr), grad.shape[0]) \
+        == F.ERR_INVALID_ARGUMENT
+
+
 def test_cold_state_force_zero_and_invariant():
     announce("cold invariance")
     ol, u, st = make_state(4, [4, 4, 4, 4], cold=True)
