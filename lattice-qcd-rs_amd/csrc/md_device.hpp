@@ -427,9 +427,13 @@ __device__ __forceinline__ void md_site(const vec2_t<S>* __restrict__ U_in, vec2
     // phase 1: the D forces (the staple sums die here), as the unscaled
     // generator combinations: the kick folds in their factors
     T f[D][8];
+    T e[D][8];
     {
         mat3<T> sig[D];
         staple_sums_site<D, S, T, R12, OWN>(U_in, g, s, nb, sig, own);
+        // the momenta of the D links: their loads run under the D products U Sigma
+#pragma unroll
+        for (int mu = 0; mu < D; ++mu) load_e<T, S, kCacheStream>(E, g.npad, mu, s, e[mu]);
 #pragma unroll
         for (int mu = 0; mu < D; ++mu) {
             const mat3<T> u = OWN ? own_get<T>(own, mu) : load_link_sel<R12, T, S>(U_in, g.npad, mu, s);
@@ -441,10 +445,8 @@ __device__ __forceinline__ void md_site(const vec2_t<S>* __restrict__ U_in, vec2
         }
     }
     // phase 2: momentum kicks of the D links
-    T e[D][8];
 #pragma unroll
     for (int mu = 0; mu < D; ++mu) {
-        load_e<T, S, kCacheStream>(E, g.npad, mu, s, e[mu]);
         if constexpr (SUMS == SUMS_BEFORE_KICK) acc->e2 += e_squares8<T>(e[mu]);
         kick_generators(f[mu], kick * fscale, e[mu]);
         if constexpr (SUMS == SUMS_AFTER_KICK) acc->e2 += e_squares8<T>(e[mu]);
